@@ -1,0 +1,251 @@
+"""bpmaxcpu_amd -- MI355X-native BPMax engine (Python binding of libbpmax_hip.so).
+
+The engine is C/HIP behind the C ABI declared in ``include/bpmax.h``; this module is a
+thin ctypes layer over it for tests, benchmarks and scripting.  It mirrors the
+reference's entry point ``bpmax(M, N, ..., seq1, seq2, FTable)``
+(TPDS/source/release/MPT_RT_V3/bpmax_complete_otf_ftable/src/bpmax.c:129) as
+``Engine.solve(seq1, seq2, ...)`` returning the F table and the optimal score.
+
+There is no CPU fallback: if the HIP library is not built, or no GPU is present,
+constructing an :class:`Engine` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbpmax_hip.so")
+REPO = os.path.dirname(HERE)
+HEADER = os.path.join(REPO, "include", "bpmax.h")
+
+OK, EINVAL, ENOMEM, EDEVICE, EWEIGHTS, EBUSY, ENOTREADY, ENOTFOUND, ETOOBIG = 0, -1, -2, -3, -4, -5, -6, -7, -8
+LAYOUT_NONE, LAYOUT_PACKED, LAYOUT_DENSE_4D = 0, 1, 2
+K_FILL, K_S1, K_OUTER, K_INNER_EXT, K_INNER_INTRA, K_CONVERT, K_R0TRI = range(7)
+KERNEL_CLASSES = ["fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri"]
+
+_ERRNAMES = {EINVAL: "EINVAL", ENOMEM: "ENOMEM", EDEVICE: "EDEVICE", EWEIGHTS: "EWEIGHTS",
+             EBUSY: "EBUSY", ENOTREADY: "ENOTREADY", ENOTFOUND: "ENOTFOUND", ETOOBIG: "ETOOBIG"}
+
+
+class BPMaxError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libbpmax_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                           "or `make -C bpmaxcpu_amd/csrc`")
+    L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    c_int, c_float, c_void_p, c_size_t = ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t
+    ip, fp = ctypes.POINTER(c_int), ctypes.POINTER(c_float)
+    ipp = ctypes.POINTER(ip)
+    sig = {
+        "bpmax_ctx_create": (c_int, [ctypes.POINTER(c_void_p), c_int]),
+        "bpmax_ctx_destroy": (c_int, [c_void_p]),
+        "bpmax_last_error": (ctypes.c_char_p, [c_void_p]),
+        "bpmax_engine_score_table": (None, [fp, fp]),
+        "bpmax_engine_solve": (c_int, [c_void_p, ip, c_int, ip, c_int, fp, fp, fp, c_int, fp]),
+        "bpmax_engine_solve_batch": (c_int, [c_void_p, c_int, ipp, ip, ipp, ip, fp, fp, fp]),
+        "bpmax_layout_bytes": (c_size_t, [c_int, c_int, c_int]),
+        "bpmax_engine_pair_device_bytes": (c_size_t, [c_int, c_int]),
+        "bpmax_tri_tiled_floats": (c_size_t, [c_int]),
+        "bpmax_r0_tri_batch_device": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+        "bpmax_tri_upload": (c_int, [c_void_p, c_int, c_int, fp, c_void_p]),
+        "bpmax_tri_download": (c_int, [c_void_p, c_int, c_int, c_void_p, fp]),
+        "bpmax_device_alloc": (c_void_p, [c_void_p, c_size_t]),
+        "bpmax_device_free": (c_int, [c_void_p, c_void_p]),
+        "bpmax_ctx_set_timing": (c_int, [c_void_p, c_int]),
+        "bpmax_ctx_kernel_stats": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_longlong),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "bpmax_ctx_reset_stats": (c_int, [c_void_p]),
+        "bpmax_ctx_stream": (c_void_p, [c_void_p]),
+        "bpmax_ctx_synchronize": (c_int, [c_void_p]),
+        "bpmax_cell_updates": (ctypes.c_double, [c_int, c_int]),
+        "bpmax_base_code": (ctypes.c_ubyte, [ctypes.c_char]),
+        "bpmax_read_sequence_file": (c_int, [ctypes.c_char_p, ip, c_int]),
+        "bpmax_write_checking_dump": (c_int, [ctypes.c_char_p, fp, c_int, c_int, c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def reference_table() -> np.ndarray:
+    """The reference's default 5x5 weight table (GC=3, AU=1, GU=1.5), from the library."""
+    L = load_library()
+    intra = np.zeros(25, np.float32)
+    inter = np.zeros(25, np.float32)
+    L.bpmax_engine_score_table(_fp(intra), _fp(inter))
+    return intra.reshape(5, 5)
+
+
+def encode(seq: str) -> np.ndarray:
+    """ACGU/T/N letters -> codes 0..4 (src/external_functions.c:74-85)."""
+    table = {"A": 0, "C": 1, "G": 2, "T": 3, "U": 3, "0": 0, "1": 1, "2": 2, "3": 3}
+    return np.array([table.get(ch.upper(), 4) for ch in seq], np.int32)
+
+
+def cell_updates(M: int, N: int) -> float:
+    """Cell updates of the full recurrence: C(M+1,3)C(N+1,3)+2C(M+1,2)C(N+1,3)+2C(M+1,3)C(N+1,2)."""
+    return float(load_library().bpmax_cell_updates(M, N))
+
+
+def _table(t, default):
+    if t is None:
+        return None if default is None else np.ascontiguousarray(default, np.float32).reshape(25)
+    return np.ascontiguousarray(t, np.float32).reshape(25)
+
+
+class Engine:
+    """One GPU context (HIP stream + device arena).  Not thread-safe."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.bpmax_ctx_create(ctypes.byref(h), device)
+        if rc != OK:
+            raise BPMaxError(rc, f"cannot create a context on HIP device {device}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bpmax_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise BPMaxError(rc, self.L.bpmax_last_error(self.h).decode(errors="replace"))
+
+    # -------------------------------------------------------------- solve
+    def solve(self, seq1, seq2, intra=None, inter=None, layout: int = LAYOUT_PACKED):
+        """Returns (F, score).  F is the packed tri-tri table (1-D float32), the dense
+        [M][M][N][N] table, or None for LAYOUT_NONE."""
+        s1 = np.ascontiguousarray(seq1, np.int32)
+        s2 = np.ascontiguousarray(seq2, np.int32)
+        M, N = len(s1), len(s2)
+        ti = _table(intra, None)
+        te = _table(inter, ti)
+        out = None
+        if layout == LAYOUT_PACKED:
+            out = np.empty((M * (M + 1) // 2) * (N * (N + 1) // 2), np.float32)
+        elif layout == LAYOUT_DENSE_4D:
+            out = np.empty((M, M, N, N), np.float32)
+        score = ctypes.c_float()
+        rc = self.L.bpmax_engine_solve(self.h, _ip(s1), M, _ip(s2), N,
+                                       None if ti is None else _fp(ti), None if te is None else _fp(te),
+                                       None if out is None else _fp(out), layout, ctypes.byref(score))
+        self._check(rc)
+        return out, float(score.value)
+
+    def solve_batch(self, seqs1: Sequence, seqs2: Sequence, intra=None, inter=None) -> np.ndarray:
+        """Scores of many independent pairs, scheduled together on the device."""
+        n = len(seqs1)
+        assert n == len(seqs2) and n > 0
+        a1 = [np.ascontiguousarray(s, np.int32) for s in seqs1]
+        a2 = [np.ascontiguousarray(s, np.int32) for s in seqs2]
+        P = ctypes.POINTER(ctypes.c_int)
+        p1 = (P * n)(*[_ip(a) for a in a1])
+        p2 = (P * n)(*[_ip(a) for a in a2])
+        Ms = np.array([len(a) for a in a1], np.int32)
+        Ns = np.array([len(a) for a in a2], np.int32)
+        ti = _table(intra, None)
+        te = _table(inter, ti)
+        scores = np.empty(n, np.float32)
+        rc = self.L.bpmax_engine_solve_batch(self.h, n, p1, _ip(Ms), p2, _ip(Ns),
+                                             None if ti is None else _fp(ti),
+                                             None if te is None else _fp(te), _fp(scores))
+        self._check(rc)
+        return scores
+
+    # ------------------------------------------------------- isolated R0 kernel
+    def tri_alloc(self, n: int, batch: int) -> int:
+        floats = self.L.bpmax_tri_tiled_floats(n) * batch
+        p = self.L.bpmax_device_alloc(self.h, floats * 4)
+        if not p:
+            raise BPMaxError(ENOMEM, self.L.bpmax_last_error(self.h).decode())
+        return p
+
+    def tri_free(self, p: int):
+        self._check(self.L.bpmax_device_free(self.h, p))
+
+    def tri_upload(self, dense: np.ndarray, dev: int):
+        d = np.ascontiguousarray(dense, np.float32)
+        batch, n, _ = d.shape
+        self._check(self.L.bpmax_tri_upload(self.h, n, batch, _fp(d), dev))
+
+    def tri_download(self, dev: int, n: int, batch: int) -> np.ndarray:
+        out = np.empty((batch, n, n), np.float32)
+        self._check(self.L.bpmax_tri_download(self.h, n, batch, dev, _fp(out)))
+        return out
+
+    def r0_tri_batch(self, n: int, batch: int, A: int, B: int, C: int):
+        self._check(self.L.bpmax_r0_tri_batch_device(self.h, n, batch, A, B, C))
+
+    # -------------------------------------------------------------- timing
+    def set_timing(self, on: bool):
+        self._check(self.L.bpmax_ctx_set_timing(self.h, 1 if on else 0))
+
+    def reset_stats(self):
+        self._check(self.L.bpmax_ctx_reset_stats(self.h))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            n = ctypes.c_longlong()
+            ms = ctypes.c_double()
+            upd = ctypes.c_double()
+            self._check(self.L.bpmax_ctx_kernel_stats(self.h, k, ctypes.byref(n), ctypes.byref(ms),
+                                                      ctypes.byref(upd)))
+            out[name] = {"launches": n.value, "ms": ms.value, "updates": upd.value}
+        return out
+
+    def synchronize(self):
+        self._check(self.L.bpmax_ctx_synchronize(self.h))
+
+    def stream(self) -> int:
+        return self.L.bpmax_ctx_stream(self.h)
+
+
+def header_functions(path: str = HEADER) -> list:
+    """Names of the functions declared in include/bpmax.h."""
+    import re
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(bpmax\w*)\s*\(", text)
+    return sorted(set(n for n in names if n not in ("bpmax_layout", "bpmax_kernel_class")))

@@ -1,0 +1,25 @@
+// bpmax_launch.h -- host-side launch wrappers of the kernels in bpmax_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bpmax_internal.h"
+
+namespace bpx {
+
+hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n,
+                        float *arena, const int *codes, const float *w);
+hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
+                            const WorkItem *items, int n, float *arena);
+hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
+                              const WorkItem *items, int n, float *arena, const int *codes, const float *w);
+hipError_t launch_s1(hipStream_t st, const PairDesc *pairs, int npairs, float *arena, const int *codes,
+                     const float *w);
+hipError_t launch_fill(hipStream_t st, float *p, long long n, float v);
+hipError_t launch_to_packed(hipStream_t st, const float *arena, const PairDesc &P, float *out);
+hipError_t launch_to_dense(hipStream_t st, const float *arena, const PairDesc &P, float *out);
+hipError_t launch_scores(hipStream_t st, const PairDesc *pairs, int npairs, const float *arena, float *scores);
+hipError_t launch_r0tri(hipStream_t st, int T, int nb, const int2 *tiles, int ntiles, int batch, const float *A,
+                        const float *B, float *C);
+hipError_t launch_dense_to_tiled(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst);
+hipError_t launch_tiled_to_dense(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst);
+
+} // namespace bpx

@@ -1,0 +1,170 @@
+/*
+ * bpmax.h -- C ABI of the MI355X BPMax engine (libbpmax_hip.so).
+ *
+ * BPMax (base-pair maximization over two RNA strands) fills the 4-D "triangle of
+ * triangles" F[i1,j1,i2,j2], 0<=i1<=j1<M over seq1 and 0<=i2<=j2<N over REVERSED seq2,
+ * and returns the optimal score F(0,M-1,0,N-1).  Every function here is plain C: no
+ * exceptions cross it, pointers are host pointers unless a name says "device", and the
+ * return value is 0 or a negative BPMAX_E* code.
+ *
+ * Reference interfaces replaced (chiranjeb/BPMaxCPU, paths relative to
+ * TPDS/source/release/MPT_RT_V3/bpmax_complete_otf_ftable/):
+ *   bpmax()                  <- src/bpmax.c:129 (prototype src/bpmax-wrapper.c:51), same
+ *                               signature, same 6-D F layout, same parameter check;
+ *   bpmax_engine_solve()     <- the same call as a library entry point with the weight
+ *                               table passed in (the reference resolves it at link time
+ *                               through e_intra_score/e_inter_score, src/external_functions.h:33-35)
+ *                               and F returned packed or dense 4-D instead of the 6-D tree;
+ *   bpmax_engine_score_table <- src/external_functions.c:21-68 (default weight table);
+ *   bpmax_write_checking_dump <- the CHECKING dump of src/bpmax-wrapper.c:318-342;
+ *   bpmax_read_sequence_file  <- ReadSequencesFromFiles, src/external_functions.c:447-468
+ *                               (returns an error on a length mismatch instead of silently
+ *                               substituting random bases).
+ */
+#ifndef BPMAX_H
+#define BPMAX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BPMAX_ABI_VERSION 1
+
+/* ---- error codes ---------------------------------------------------------------- */
+#define BPMAX_OK 0
+#define BPMAX_EINVAL -1      /* bad argument (lengths, codes, layout, null pointer)   */
+#define BPMAX_ENOMEM -2      /* device or host allocation failed                      */
+#define BPMAX_EDEVICE -3     /* HIP runtime error                                     */
+#define BPMAX_EWEIGHTS -4    /* weight table not exact (see bpmax_engine_solve)       */
+#define BPMAX_EBUSY -5       /* serving queue full / admission refused                */
+#define BPMAX_ENOTREADY -6   /* request not finished yet                              */
+#define BPMAX_ENOTFOUND -7   /* unknown request id                                    */
+#define BPMAX_ETOOBIG -8     /* pair can never fit the device memory budget           */
+
+/* ---- F output layouts -------------------------------------------------------------- */
+typedef enum {
+    BPMAX_LAYOUT_NONE = 0,       /* score only; F stays on the device                  */
+    BPMAX_LAYOUT_PACKED_TRI_TRI, /* tri(M)*tri(N) floats, order i1, j1>=i1, i2, j2>=i2
+                                    (the order of the reference's CHECKING dump)        */
+    BPMAX_LAYOUT_DENSE_4D        /* [M][M][N][N] floats (IPDPSW-style 4-D table,
+                                    IPDPSW/source/.../src/bpmax-wrapper.c:175); only the
+                                    cells with i1<=j1 and i2<=j2 are written            */
+} bpmax_layout;
+
+typedef struct bpmax_ctx bpmax_ctx; /* one HIP stream + device arena on one GPU */
+
+/* Sequence codes: 0..4 = A, C, G, U/T, N (src/external_functions.c:74-85). */
+unsigned char bpmax_base_code(char c);
+
+/* Weight tables are 5x5 row-major float arrays, w[5*x+y] = score of pairing code x with
+ * code y.  Fills the reference's hard-coded table (GC=3, AU=1, GU=1.5, everything else 0,
+ * identical for intra- and inter-strand pairs). */
+void bpmax_engine_score_table(float intra[25], float inter[25]);
+
+/* Context on HIP device `device`.  Not thread-safe: one host thread per context. */
+int bpmax_ctx_create(bpmax_ctx **out, int device);
+int bpmax_ctx_destroy(bpmax_ctx *ctx);
+const char *bpmax_last_error(const bpmax_ctx *ctx); /* human-readable, never NULL */
+
+/* Solve one pair.
+ *   seq1[M], seq2[N]: codes 0..4, seq2 in its natural 5'->3' order (not reversed).
+ *   intra, inter: 5x5 weight tables; NULL selects the reference table.  Weights must be
+ *     finite, >= 0 and exact multiples of 2^-8 with every reachable score below 2^22, so
+ *     max-plus evaluation is exact in any order (BPMAX_EWEIGHTS otherwise).
+ *   F_out: host buffer for `layout`, or NULL with BPMAX_LAYOUT_NONE.
+ *   score_out: receives F(0,M-1,0,N-1).
+ * M >= 1, N >= 1. */
+int bpmax_engine_solve(bpmax_ctx *ctx, const int *seq1, int M, const int *seq2, int N,
+                       const float *intra, const float *inter, float *F_out,
+                       bpmax_layout layout, float *score_out);
+
+/* Bytes that `layout` needs for an MxN pair on the host. */
+size_t bpmax_layout_bytes(int M, int N, bpmax_layout layout);
+
+/* Device bytes the engine holds for one resident MxN pair (F in its tiled layout plus
+ * the single-strand tables). */
+size_t bpmax_engine_pair_device_bytes(int M, int N);
+
+/* ---- batched solve ----------------------------------------------------------------- */
+/* Solve `npairs` independent pairs of any sizes in one scheduled pass on the device.
+ * seq1s[p] / seq2s[p] point to codes of length Ms[p] / Ns[p].  scores[p] receives each
+ * optimal score.  F tables stay on the device (score-only).  Weights as in
+ * bpmax_engine_solve, shared by all pairs. */
+int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s,
+                             const int *Ms, const int *const *seq2s, const int *Ns,
+                             const float *intra, const float *inter, float *scores);
+
+/* ---- drop-in compatibility symbol ----------------------------------------------- */
+/* Exact signature of the reference entry point (src/bpmax.c:129).  Computes on the GPU
+ * and writes the caller's 6-level F pointer tree
+ *   FTable[i1][j1][(i2+R)/N_tile][(j2+R)/N_tile][(i2+R)%N_tile][(j2+R)%N_tile]
+ * for every i1<=j1, i2<=j2 (src/bpmax-wrapper.c:44).  The weight table is read through
+ * the linked e_intra_score()/e_inter_score() plug point when those symbols are present in
+ * the process, else the reference table is used.  Invalid parameters print the
+ * reference's message and exit(-1), as the reference does (src/bpmax.c:131-133); MR, NR
+ * and P are validated but the GPU schedule does not use them. */
+void bpmax(long M, long N, long N_sec, long N_tile, long R, long MR, long NR, long P,
+           int *seq1, int *seq2, float ******FTable);
+
+/* ---- file formats -------------------------------------------------------------------- */
+/* Reads the first FASTA-like record of `path` (header lines start with '>', letters are
+ * mapped by bpmax_base_code).  Writes at most `cap` codes to out and returns the record
+ * length, or a negative error.  Unlike the reference reader, a length different from the
+ * expected one is reported to the caller instead of being replaced by random bases. */
+int bpmax_read_sequence_file(const char *path, int *out, int cap);
+
+/* Writes "FTable(i1,j1,i2,j2)=%0.2f\n" lines for every defined cell of a packed table,
+ * in the reference's CHECKING order (src/bpmax-wrapper.c:318-342).  with_prompt=0 writes
+ * only the values (the NO_PROMPT form). */
+int bpmax_write_checking_dump(const char *path, const float *F_packed, int M, int N,
+                              int with_prompt);
+
+/* ---- isolated R0 max-plus kernel ------------------------------------------------------ */
+/* For `batch` independent upper-triangular n x n matrix triples held on the DEVICE in the
+ * engine's tiled layout (see bpmax_tri_tiled_floats), computes
+ *   C[i][j] = max(C[i][j], max_{i<=k<=j} A[i][k] + B[k][j])  for i<=j.
+ * A, B, C are device pointers, each holding batch*bpmax_tri_tiled_floats(n) floats. */
+size_t bpmax_tri_tiled_floats(int n);
+int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const float *A_dev,
+                              const float *B_dev, float *C_dev);
+/* Host <-> tiled-device conversion helpers for those triples (dense row-major n x n on
+ * the host; entries below the diagonal are ignored on upload, -inf on download). */
+int bpmax_tri_upload(bpmax_ctx *ctx, int n, int batch, const float *host_dense, float *dev_tiled);
+int bpmax_tri_download(bpmax_ctx *ctx, int n, int batch, const float *dev_tiled, float *host_dense);
+void *bpmax_device_alloc(bpmax_ctx *ctx, size_t bytes);
+int bpmax_device_free(bpmax_ctx *ctx, void *p);
+
+/* ---- timing / introspection --------------------------------------------------------- */
+/* When enabled, every kernel launch is bracketed by HIP events on the context stream and
+ * per-kernel-class totals are accumulated (adds host overhead; off by default). */
+int bpmax_ctx_set_timing(bpmax_ctx *ctx, int enable);
+/* Kernel classes for bpmax_ctx_kernel_stats. */
+typedef enum {
+    BPMAX_K_FILL = 0,
+    BPMAX_K_S1,
+    BPMAX_K_OUTER,      /* R0 + init + P1 + R3 + R4 (one launch per outer diagonal) */
+    BPMAX_K_INNER_EXT,  /* R1/R2 cross-tile products (and S2's)                       */
+    BPMAX_K_INNER_INTRA,/* intra-tile wavefront (R1/R2/P2 inside a tile; S2's)       */
+    BPMAX_K_CONVERT,
+    BPMAX_K_R0TRI,      /* isolated R0 kernel                                          */
+    BPMAX_K_NCLASSES
+} bpmax_kernel_class;
+/* Returns launches, total milliseconds and total algorithmic cell updates for a class
+ * since the last reset. */
+int bpmax_ctx_kernel_stats(bpmax_ctx *ctx, int kclass, long long *launches, double *ms,
+                           double *updates);
+int bpmax_ctx_reset_stats(bpmax_ctx *ctx);
+/* The hipStream_t the context launches on (as void*). */
+void *bpmax_ctx_stream(bpmax_ctx *ctx);
+int bpmax_ctx_synchronize(bpmax_ctx *ctx);
+/* Cell updates of the full recurrence for an MxN pair:
+ *   C(M+1,3)C(N+1,3) + 2C(M+1,2)C(N+1,3) + 2C(M+1,3)C(N+1,2)  (R0 + R1/R2 + R3/R4). */
+double bpmax_cell_updates(int M, int N);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BPMAX_H */

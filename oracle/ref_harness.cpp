@@ -1,0 +1,169 @@
+// ref_harness.cpp -- drives the reference's own bpmax() (compiled from the reference
+// sources by oracle/Makefile into oracle/_ref/) on caller-chosen sequences and weight
+// tables.  TEST INFRASTRUCTURE ONLY: it writes golden vectors and times the CPU baseline.
+//
+// It replaces the reference's AlphaZ "external functions" plug point
+// (src/external_functions.h:33-35, defined in src/external_functions.c:8-68) with a
+// table read from the input file, so the same reference objects serve every weight
+// table.  The F table is allocated exactly as the reference driver does
+// (src/bpmax-wrapper.c:158-197: one contiguous M*M*N_sec*N_sec*(N_tile+1)*N_tile buffer
+// under a 6-level pointer tree), then canonicalized to the packed tri-tri order of the
+// CHECKING dump (src/bpmax-wrapper.c:318-342).  Denormals are flushed on every OpenMP
+// thread before the call: the reference seeds some reductions with the smallest
+// denormal (src/bpmax_inner_reductions_diagonal_tile.c:144-145) and only its FTZ build
+// passes its own verify.
+//
+// Input file (little-endian):  int32 magic 0x42504d58, int32 M, int32 N,
+//   int32 seq1[M], int32 seq2[N], float32 intra[25], float32 inter[25]
+// Output file: float32 score, float64 seconds(median), int32 repeats, int32 N_tile,
+//   then (unless --no-table) float32 F[tri(M)*tri(N)].
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+#include <omp.h>
+#include <sys/time.h>
+#include <xmmintrin.h>
+#include <pmmintrin.h>
+
+void bpmax(long, long, long, long, long, long, long, long, int *, int *, float ******);
+
+static float g_intra[25], g_inter[25];
+float e_intra_score(int a, int b) { return g_intra[5 * a + b]; }
+float e_inter_score(int a, int b) { return g_inter[5 * a + b]; }
+float e_get_minimum(int, int) { return -FLT_MAX; }
+void Dump2D(int, float **, const char *, int) {}
+void Dump4D(int, int, int, float ****, const char *) {}
+
+static double now_s()
+{
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + tv.tv_usec * 1e-6;
+}
+
+static void die(const char *m)
+{
+    fprintf(stderr, "ref_harness: %s\n", m);
+    exit(2);
+}
+
+// Section size for the reference: N_tile must be a multiple of the register tile's NR,
+// give at least 2 sections (parameter check src/bpmax.c:131) and pad R = N_sec*N_tile-N
+// < N_tile.  R is computed here for every case, including N % N_tile == 0 where the
+// reference driver leaves it uninitialized (src/bpmax-wrapper.c:158-167).
+static void pick_tile(long N, long &N_tile, long &MR, long &NR)
+{
+    const long cands[][3] = {{48, 3, 24}, {24, 3, 24}, {16, 2, 8}, {8, 1, 8}};
+    for (auto &c : cands) {
+        long nsec = (N + c[0] - 1) / c[0];
+        if (nsec >= 2) { N_tile = c[0]; MR = c[1]; NR = c[2]; return; }
+    }
+    die("N too small for the reference (needs N > 8)");
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s in.bin out.bin [--no-table] [--repeat R] [--ntile T --mr MR --nr NR]\n", argv[0]);
+        return 2;
+    }
+    bool table = true;
+    int repeat = 1;
+    long N_tile = 0, MR = 0, NR = 0;
+    for (int a = 3; a < argc; ++a) {
+        if (!strcmp(argv[a], "--no-table")) table = false;
+        else if (!strcmp(argv[a], "--repeat") && a + 1 < argc) repeat = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "--ntile") && a + 1 < argc) N_tile = atol(argv[++a]);
+        else if (!strcmp(argv[a], "--mr") && a + 1 < argc) MR = atol(argv[++a]);
+        else if (!strcmp(argv[a], "--nr") && a + 1 < argc) NR = atol(argv[++a]);
+        else die("bad argument");
+    }
+    FILE *fi = fopen(argv[1], "rb");
+    if (!fi) die("cannot open input");
+    int32_t hdr[3];
+    if (fread(hdr, 4, 3, fi) != 3 || hdr[0] != 0x42504d58) die("bad input header");
+    const long M = hdr[1], N = hdr[2];
+    std::vector<int> seq1(M), seq2(N);
+    if (fread(seq1.data(), 4, M, fi) != (size_t)M || fread(seq2.data(), 4, N, fi) != (size_t)N ||
+        fread(g_intra, 4, 25, fi) != 25 || fread(g_inter, 4, 25, fi) != 25)
+        die("short input");
+    fclose(fi);
+    if (N_tile == 0) pick_tile(N, N_tile, MR, NR);
+    const long N_sec = (N + N_tile - 1) / N_tile;
+    const long R = N_sec * N_tile - N;
+    const long P = std::min<long>(omp_get_max_threads(), 128);
+
+    const size_t sec = (size_t)(N_tile + 1) * N_tile;
+    const size_t total = (size_t)M * M * N_sec * N_sec * sec;
+    float *lin = (float *)malloc(sizeof(float) * total);
+    if (!lin) die("F allocation failed");
+    // pointer tree exactly as src/bpmax-wrapper.c:175-197
+    float ******F = (float ******)malloc(sizeof(float *****) * M);
+    for (long a = 0; a < M; ++a) {
+        F[a] = (float *****)malloc(sizeof(float ****) * M);
+        for (long b = 0; b < M; ++b) {
+            F[a][b] = (float ****)malloc(sizeof(float ***) * N_sec);
+            for (long c = 0; c < N_sec; ++c) {
+                F[a][b][c] = (float ***)malloc(sizeof(float **) * N_sec);
+                for (long d = 0; d < N_sec; ++d) {
+                    F[a][b][c][d] = (float **)malloc(sizeof(float *) * (N_tile + 1));
+                    for (long e = 0; e < N_tile + 1; ++e)
+                        F[a][b][c][d][e] = lin + (((a * M + b) * N_sec + c) * N_sec + d) * sec + e * N_tile;
+                }
+            }
+        }
+    }
+
+    // flush denormals on the caller and on every OpenMP worker
+    _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+    _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+#pragma omp parallel
+    {
+        _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+        _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+    }
+
+    std::vector<double> times;
+    for (int r = 0; r < std::max(1, repeat); ++r) {
+        double t0 = now_s();
+        bpmax(M, N, N_sec, N_tile, R, MR, NR, P, seq1.data(), seq2.data(), F);
+        times.push_back(now_s() - t0);
+    }
+    std::vector<double> sorted = times;
+    std::sort(sorted.begin(), sorted.end());
+    const double med = sorted[sorted.size() / 2];
+#define FCELL(i1, j1, i2, j2) \
+    F[i1][j1][((i2) + R) / N_tile][((j2) + R) / N_tile][((i2) + R) % N_tile][((j2) + R) % N_tile]
+    float score = FCELL(0, M - 1, 0, N - 1);
+
+    FILE *fo = fopen(argv[2], "wb");
+    if (!fo) die("cannot open output");
+    int32_t rep = (int32_t)times.size(), nt = (int32_t)N_tile;
+    fwrite(&score, 4, 1, fo);
+    fwrite(&med, 8, 1, fo);
+    fwrite(&rep, 4, 1, fo);
+    fwrite(&nt, 4, 1, fo);
+    if (table) {
+        std::vector<float> row(N);
+        for (long i1 = 0; i1 < M; ++i1)
+            for (long j1 = i1; j1 < M; ++j1)
+                for (long i2 = 0; i2 < N; ++i2) {
+                    long n = 0;
+                    for (long j2 = i2; j2 < N; ++j2) {
+                        float v = FCELL(i1, j1, i2, j2);
+                        if (std::fpclassify(v) == FP_SUBNORMAL) v = 0.0f;
+                        row[n++] = v;
+                    }
+                    fwrite(row.data(), 4, n, fo);
+                }
+    }
+    fclose(fo);
+    printf("ref_harness: M=%ld N=%ld N_tile=%ld N_sec=%ld R=%ld MR=%ld NR=%ld P=%ld score=%.2f median=%.4fs\n",
+           M, N, N_tile, N_sec, R, MR, NR, P, score, med);
+    return 0;
+}

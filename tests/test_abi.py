@@ -1,0 +1,94 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function include/bpmax.h
+declares, and its host-only helpers (closed forms, sequence reader, CHECKING dump) match
+the reference's conventions."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import bpmaxcpu_amd as B
+from oracle import oracle as O
+from tests import golden_io as G
+
+
+def test_library_exports_every_header_function():
+    L = B.load_library()
+    names = B.header_functions()
+    assert "bpmax" in names and "bpmax_engine_solve" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_cell_update_closed_form():
+    # values quoted with the workload descriptions: 64x512 -> 1.0816e12, 16x100 -> 1.6552e8
+    assert abs(B.cell_updates(64, 512) / 1.0816e12 - 1) < 1e-4
+    assert abs(B.cell_updates(16, 100) / 1.6552e8 - 1) < 1e-4
+    # brute-force count of the recurrence's terms on a tiny shape
+    M, N = 5, 7
+    n = 0
+    for i1 in range(M):
+        for j1 in range(i1, M):
+            for i2 in range(N):
+                for j2 in range(i2, N):
+                    n += (j1 - i1) * (j2 - i2) + 2 * (j2 - i2) + 2 * (j1 - i1)
+    assert B.cell_updates(M, N) == n
+
+
+def test_layout_bytes():
+    L = B.load_library()
+    assert L.bpmax_layout_bytes(64, 512, B.LAYOUT_PACKED) == 2080 * 131328 * 4
+    assert L.bpmax_layout_bytes(3, 5, B.LAYOUT_DENSE_4D) == 3 * 3 * 5 * 5 * 4
+    assert L.bpmax_layout_bytes(0, 5, B.LAYOUT_PACKED) == 0
+
+
+def test_base_codes_and_default_table():
+    L = B.load_library()
+    assert [L.bpmax_base_code(c.encode()) for c in "ACGUTNacgut0123X"] == [0, 1, 2, 3, 3, 4, 0, 1, 2, 3, 3, 0, 1, 2, 3, 4]
+    assert np.array_equal(B.reference_table(), O.reference_table())
+
+
+def test_read_sequence_file(tmp_path):
+    L = B.load_library()
+    p = tmp_path / "s.fa"
+    p.write_text(">name one\nACGU\nacgt nX\n>second\nGGGG\n")
+    out = np.zeros(16, np.int32)
+    n = L.bpmax_read_sequence_file(str(p).encode(), B._ip(out), 16)
+    assert n == 10
+    assert out[:10].tolist() == [0, 1, 2, 3, 0, 1, 2, 3, 4, 4]
+    p.write_text("AC;GU\n")
+    assert L.bpmax_read_sequence_file(str(p).encode(), B._ip(out), 16) == 2
+    assert L.bpmax_read_sequence_file(b"/nonexistent/file", B._ip(out), 16) == B.EINVAL
+
+
+def test_checking_dump_matches_reference_format(tmp_path):
+    """The dump writer reproduces the reference's CHECKING output byte-for-byte
+    (src/bpmax-wrapper.c:318-342), checked against oracle/_ref/bpmax.check when built."""
+    L = B.load_library()
+    fx = G.load_full(os.path.join(G.GOLDEN, "full_5x17_ref.npz"))
+    M, N = len(fx["seq1"]), len(fx["seq2"])
+    out = tmp_path / "dump.txt"
+    F = np.ascontiguousarray(fx["F"], np.float32)
+    assert L.bpmax_write_checking_dump(str(out).encode(), B._fp(F), M, N, 1) == 0
+    lines = out.read_text().splitlines()
+    assert len(lines) == O.tri(M) * O.tri(N)
+    assert lines[0].startswith("FTable(0,0,0,0)=")
+    if not os.path.exists(O.REF_CHECK):
+        pytest.skip("oracle/_ref/bpmax.check not built")
+    stdin = " ".join(str(int(x)) for x in list(fx["seq1"]) + list(fx["seq2"])) + "\n"
+    r = subprocess.run([O.REF_CHECK, str(M), str(N), "16", "2", "8"], input=stdin, capture_output=True,
+                       text=True, check=True)
+    ref_lines = [ln[ln.index("FTable("):] for ln in r.stdout.splitlines() if "FTable(" in ln]
+    assert ref_lines == lines
+
+
+def test_context_without_gpu_fails_cleanly():
+    import ctypes
+    L = B.load_library()
+    h = ctypes.c_void_p()
+    rc = L.bpmax_ctx_create(ctypes.byref(h), 0)
+    if rc == B.OK:
+        L.bpmax_ctx_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == B.EDEVICE
+    assert L.bpmax_ctx_create(None, 0) == B.EINVAL

@@ -1,0 +1,234 @@
+"""GPU parity: the HIP engine against the reference's golden vectors and the CPU oracle,
+bit-for-bit.  Every test calls through the C ABI (libbpmax_hip.so via ctypes)."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import bpmaxcpu_amd as B
+from oracle import oracle as O
+from tests import golden_io as G
+
+pytestmark = pytest.mark.gpu
+
+FULL = G.full_fixtures()
+HASHED = G.hash_fixtures()
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = B.Engine(0)
+    yield e
+    e.close()
+
+
+def _bits_equal(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("path", FULL, ids=[G.fid(p) for p in FULL])
+def test_full_fixture(eng, path):
+    fx = G.load_full(path)
+    _say(f"full fixture {G.fid(path)}")
+    F, score = eng.solve(fx["seq1"], fx["seq2"], fx["intra"], fx["inter"])
+    assert score == fx["score"]
+    assert _bits_equal(F, fx["F"])
+
+
+@pytest.mark.parametrize("path", HASHED, ids=[G.fid(p) for p in HASHED])
+def test_hash_fixture(eng, path):
+    fx = G.load_hash(path)
+    _say(f"hash fixture {G.fid(path)} M={fx['M']} N={fx['N']}")
+    F, score = eng.solve(fx["seq1"], fx["seq2"], fx["intra"], fx["inter"])
+    assert score == fx["score"]
+    for i1, j1, i2, j2, v in fx["samples"]:
+        assert F[O.packed_index(i1, j1, i2, j2, fx["M"], fx["N"])] == v
+    assert G.sha256_f32(F) == fx["sha256"]
+
+
+# shapes around the tile edges (T = 32 for N <= 64, 64 above), the P2/P1 hairpin
+# thresholds (d <= 3), single-base strands and N below the reference's own minimum of 8
+EDGE = [(1, 1), (1, 2), (2, 1), (1, 5), (5, 1), (3, 4), (4, 5), (6, 8), (2, 31), (3, 32), (2, 33),
+        (1, 64), (2, 65), (3, 63), (7, 64), (2, 96), (2, 127), (2, 128), (3, 129), (9, 70)]
+
+
+@pytest.mark.parametrize("M,N", EDGE)
+def test_edge_shapes_vs_oracle(eng, M, N):
+    rng = np.random.default_rng(1000 * M + N)
+    s1 = rng.integers(0, 5, M).astype(np.int32)   # includes code 4 (N, never pairs)
+    s2 = rng.integers(0, 5, N).astype(np.int32)
+    intra, inter = O.table_asym()
+    _say(f"edge M={M} N={N}")
+    F, score = eng.solve(s1, s2, intra, inter)
+    Fo, so = O.bpmax(s1, s2, intra, inter)
+    assert score == so
+    assert _bits_equal(F, Fo)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_shapes_vs_oracle(eng, seed):
+    rng = np.random.default_rng(seed)
+    M = int(rng.integers(2, 14))
+    N = int(rng.integers(20, 200))
+    s1 = rng.integers(0, 4, M).astype(np.int32)
+    s2 = rng.integers(0, 4, N).astype(np.int32)
+    tab = O.table_321() if seed % 2 else O.reference_table()
+    _say(f"random seed={seed} M={M} N={N}")
+    F, score = eng.solve(s1, s2, tab, tab)
+    Fo, so = O.bpmax(s1, s2, tab, tab)
+    assert score == so and _bits_equal(F, Fo)
+
+
+def test_dense_layout_matches_packed(eng):
+    rng = np.random.default_rng(5)
+    M, N = 5, 70
+    s1, s2 = rng.integers(0, 4, M).astype(np.int32), rng.integers(0, 4, N).astype(np.int32)
+    _say("dense layout 5x70")
+    Fp, sp = eng.solve(s1, s2, layout=B.LAYOUT_PACKED)
+    Fd, sd = eng.solve(s1, s2, layout=B.LAYOUT_DENSE_4D)
+    _, sn = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    assert sp == sd == sn
+    k = 0
+    for i1 in range(M):
+        for j1 in range(M):
+            if j1 < i1:
+                assert np.all(Fd[i1, j1] == 0)
+                continue
+            for i2 in range(N):
+                row = Fp[k:k + N - i2]
+                k += N - i2
+                assert _bits_equal(Fd[i1, j1, i2, i2:], row)
+                assert np.all(Fd[i1, j1, i2, :i2] == 0)
+
+
+def test_batch_matches_single_and_oracle(eng):
+    rng = np.random.default_rng(77)
+    pairs = [(int(rng.integers(1, 12)), int(rng.integers(1, 150))) for _ in range(12)]
+    s1s = [rng.integers(0, 4, m).astype(np.int32) for m, _ in pairs]
+    s2s = [rng.integers(0, 4, n).astype(np.int32) for _, n in pairs]
+    _say(f"batch of {len(pairs)} mixed pairs")
+    scores = eng.solve_batch(s1s, s2s)
+    for p in range(len(pairs)):
+        _, so = O.bpmax(s1s[p], s2s[p])
+        _, ss = eng.solve(s1s[p], s2s[p], layout=B.LAYOUT_NONE)
+        assert scores[p] == so == ss, (p, pairs[p])
+
+
+def test_invalid_inputs_are_rejected(eng):
+    _say("invalid inputs")
+    with pytest.raises(B.BPMaxError) as e:
+        eng.solve(np.array([0, 1, 7], np.int32), np.array([0, 1, 2], np.int32))
+    assert e.value.code == B.EINVAL
+    with pytest.raises(B.BPMaxError) as e:
+        eng.solve(np.zeros(0, np.int32), np.array([0, 1, 2], np.int32))
+    assert e.value.code == B.EINVAL
+    bad = O.reference_table().copy()
+    bad[0, 3] = -1.0
+    with pytest.raises(B.BPMaxError) as e:
+        eng.solve(np.array([0, 1, 2], np.int32), np.array([3, 2, 1], np.int32), bad)
+    assert e.value.code == B.EWEIGHTS
+    bad[0, 3] = 0.1
+    with pytest.raises(B.BPMaxError) as e:
+        eng.solve(np.array([0, 1, 2], np.int32), np.array([3, 2, 1], np.int32), bad)
+    assert e.value.code == B.EWEIGHTS
+    # the context is still usable afterwards
+    _, s = eng.solve(np.array([0], np.int32), np.array([3], np.int32))
+    assert s == 1.0
+
+
+@pytest.mark.parametrize("n,batch", [(1, 1), (5, 3), (64, 2), (65, 2), (130, 3), (512, 2)])
+def test_r0_isolated_vs_oracle(eng, n, batch):
+    rng = np.random.default_rng(n + batch)
+    up = np.triu(np.ones((n, n), bool))
+    mk = lambda: np.where(up, rng.integers(0, 201, (batch, n, n)), -np.inf).astype(np.float32)  # noqa: E731
+    A, Bm, C = mk(), mk(), mk()
+    _say(f"r0 isolated n={n} batch={batch}")
+    dA, dB, dC = eng.tri_alloc(n, batch), eng.tri_alloc(n, batch), eng.tri_alloc(n, batch)
+    try:
+        eng.tri_upload(A, dA)
+        eng.tri_upload(Bm, dB)
+        eng.tri_upload(C, dC)
+        eng.r0_tri_batch(n, batch, dA, dB, dC)
+        got = eng.tri_download(dC, n, batch)
+    finally:
+        for p in (dA, dB, dC):
+            eng.tri_free(p)
+    for m in range(batch):
+        want = O.maxplus_tri(A[m], Bm[m], C[m])
+        assert _bits_equal(np.where(up, got[m], 0), np.where(up, want, 0))
+        assert np.all(got[m][~up] == -np.inf)
+
+
+def _tree6(M, N, N_tile):
+    """The reference driver's 6-level pointer tree over one contiguous buffer
+    (src/bpmax-wrapper.c:175-197), built with ctypes."""
+    N_sec = (N + N_tile - 1) // N_tile
+    R = N_sec * N_tile - N
+    sec = (N_tile + 1) * N_tile
+    lin = np.full(M * M * N_sec * N_sec * sec, np.nan, np.float32)
+    base = lin.ctypes.data
+    FP = ctypes.POINTER(ctypes.c_float)
+    keep = []
+
+    def arr(t, n):
+        a = (t * n)()
+        keep.append(a)
+        return a
+
+    L5 = ctypes.POINTER(FP)
+    L4 = ctypes.POINTER(L5)
+    L3 = ctypes.POINTER(L4)
+    L2 = ctypes.POINTER(L3)
+    top = arr(L2, M)
+    for a in range(M):
+        t1 = arr(L3, M)
+        for b in range(M):
+            t2 = arr(L4, N_sec)
+            for c in range(N_sec):
+                t3 = arr(L5, N_sec)
+                for d in range(N_sec):
+                    t4 = arr(FP, N_tile + 1)
+                    off = (((a * M + b) * N_sec + c) * N_sec + d) * sec
+                    for e in range(N_tile + 1):
+                        t4[e] = ctypes.cast(base + 4 * (off + e * N_tile), FP)
+                    t3[d] = ctypes.cast(t4, L5)
+                t2[c] = ctypes.cast(t3, L4)
+            t1[b] = ctypes.cast(t2, L3)
+        top[a] = ctypes.cast(t1, L2)
+    return lin, top, keep, N_sec, R
+
+
+def test_compat_bpmax_symbol_writes_6d_tree(eng):
+    """The V3-signature entry point fills the reference's 6-D layout."""
+    fx = G.load_full(os.path.join(G.GOLDEN, "full_16x100_ref.npz"))
+    M, N, N_tile = 16, 100, 48
+    lin, top, keep, N_sec, R = _tree6(M, N, N_tile)
+    L = B.load_library()
+    fn = L.bpmax
+    fn.restype = None
+    s1 = np.ascontiguousarray(fx["seq1"], np.int32)
+    s2 = np.ascontiguousarray(fx["seq2"], np.int32)
+    _say("compat bpmax() 16x100 6-D tree")
+    fn(ctypes.c_long(M), ctypes.c_long(N), ctypes.c_long(N_sec), ctypes.c_long(N_tile), ctypes.c_long(R),
+       ctypes.c_long(3), ctypes.c_long(24), ctypes.c_long(8), B._ip(s1), B._ip(s2), top)
+    sec = (N_tile + 1) * N_tile
+    F = fx["F"]
+    k = 0
+    for i1 in range(M):
+        for j1 in range(i1, M):
+            for i2 in range(N):
+                for j2 in range(i2, N):
+                    ci, cj = i2 + R, j2 + R
+                    off = (((i1 * M + j1) * N_sec + ci // N_tile) * N_sec + cj // N_tile) * sec
+                    v = lin[off + (ci % N_tile) * N_tile + cj % N_tile]
+                    assert v == F[k], (i1, j1, i2, j2)
+                    k += 1
