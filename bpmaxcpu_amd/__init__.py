@@ -18,14 +18,17 @@ from typing import Iterable, Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbpmax_hip.so")
+# BPMAX_LIB selects another build of the library (A/B comparisons of kernel variants)
+LIB_PATH = os.environ.get("BPMAX_LIB") or os.path.join(HERE, "libbpmax_hip.so")
 REPO = os.path.dirname(HERE)
 HEADER = os.path.join(REPO, "include", "bpmax.h")
 
 OK, EINVAL, ENOMEM, EDEVICE, EWEIGHTS, EBUSY, ENOTREADY, ENOTFOUND, ETOOBIG = 0, -1, -2, -3, -4, -5, -6, -7, -8
 LAYOUT_NONE, LAYOUT_PACKED, LAYOUT_DENSE_4D = 0, 1, 2
-K_FILL, K_S1, K_OUTER, K_INNER_EXT, K_INNER_INTRA, K_CONVERT, K_R0TRI = range(7)
-KERNEL_CLASSES = ["fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri"]
+K_FILL, K_S1, K_OUTER, K_INNER_EXT, K_INNER_INTRA, K_CONVERT, K_R0TRI, K_TASKS = range(8)
+KERNEL_CLASSES = ["fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri", "tasks"]
+SCHED_TASKS, SCHED_LAUNCH, SCHED_PIPE = 0, 1, 2
+SCHEDULES = {"tasks": SCHED_TASKS, "launch": SCHED_LAUNCH, "pipe": SCHED_PIPE}
 
 _ERRNAMES = {EINVAL: "EINVAL", ENOMEM: "ENOMEM", EDEVICE: "EDEVICE", EWEIGHTS: "EWEIGHTS",
              EBUSY: "EBUSY", ENOTREADY: "ENOTREADY", ENOTFOUND: "ENOTFOUND", ETOOBIG: "ETOOBIG"}
@@ -71,6 +74,8 @@ def load_library(path: str = LIB_PATH):
         "bpmax_ctx_kernel_stats": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_longlong),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "bpmax_ctx_reset_stats": (c_int, [c_void_p]),
+        "bpmax_ctx_set_schedule": (c_int, [c_void_p, c_int, c_int]),
+        "bpmax_ctx_set_graphs": (c_int, [c_void_p, c_int]),
         "bpmax_ctx_stream": (c_void_p, [c_void_p]),
         "bpmax_ctx_synchronize": (c_int, [c_void_p]),
         "bpmax_cell_updates": (ctypes.c_double, [c_int, c_int]),
@@ -217,7 +222,19 @@ class Engine:
     def r0_tri_batch(self, n: int, batch: int, A: int, B: int, C: int):
         self._check(self.L.bpmax_r0_tri_batch_device(self.h, n, batch, A, B, C))
 
-    # -------------------------------------------------------------- timing
+    # -------------------------------------------------------------- schedule / timing
+    def set_schedule(self, schedule, chunk: int = 0):
+        """SCHED_PIPE (default: per-phase launches on two streams, graph-replayed),
+        SCHED_LAUNCH (the same launches on one stream) or SCHED_TASKS (one dependency-tile
+        scheduler launch per solve), or their names; chunk = K-steps per task."""
+        if isinstance(schedule, str):
+            schedule = SCHEDULES[schedule]
+        self._check(self.L.bpmax_ctx_set_schedule(self.h, schedule, chunk))
+
+    def set_graphs(self, on: bool):
+        """Capture and replay the pipelined schedule as a HIP graph (default off)."""
+        self._check(self.L.bpmax_ctx_set_graphs(self.h, 1 if on else 0))
+
     def set_timing(self, on: bool):
         self._check(self.L.bpmax_ctx_set_timing(self.h, 1 if on else 0))
 

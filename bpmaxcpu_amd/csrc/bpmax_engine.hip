@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,7 +28,7 @@ using namespace bpx;
 
 namespace bpx {
 
-const char *kClassNames[BPMAX_K_NCLASSES] = {"fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri"};
+const char *kClassNames[BPMAX_K_NCLASSES] = {"fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri", "tasks"};
 
 int choose_tile(int N) { return N <= 64 ? 32 : 64; }
 
@@ -127,7 +128,19 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
     if (hipSetDevice(device) != hipSuccess) return BPMAX_EDEVICE;
     bpmax_ctx *c = new bpmax_ctx();
     c->device = device;
+    if (const char *e = getenv("BPMAX_SCHEDULE")) {
+        if (strcmp(e, "launch") == 0) c->schedule = BPMAX_SCHED_LAUNCH;
+        else if (strcmp(e, "tasks") == 0) c->schedule = BPMAX_SCHED_TASKS;
+        else c->schedule = BPMAX_SCHED_PIPE;
+    }
+    if (const char *e = getenv("BPMAX_TASK_CHUNK")) c->task_chunk = std::max(1, atoi(e));
+    if (const char *e = getenv("BPMAX_GRAPHS")) c->graphs = atoi(e) != 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BPMAX_EDEVICE;
+    }
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return BPMAX_EDEVICE;
     }
@@ -147,8 +160,13 @@ extern "C" int bpmax_ctx_destroy(bpmax_ctx *ctx)
     free_buf(ctx->out);
     free_buf(ctx->scores);
     free_buf(ctx->r0meta);
+    free_buf(ctx->flags);
+    (void)hipStreamSynchronize(ctx->stream2);
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto &e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto &e : ctx->sync_ev) (void)hipEventDestroy(e);
     for (void *p : ctx->user_allocs) (void)hipFree(p);
+    (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BPMAX_OK;
@@ -172,6 +190,23 @@ extern "C" int bpmax_ctx_set_timing(bpmax_ctx *ctx, int enable)
 {
     if (!ctx) return BPMAX_EINVAL;
     ctx->timing = enable != 0;
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_ctx_set_schedule(bpmax_ctx *ctx, int schedule, int chunk)
+{
+    if (!ctx || chunk < 0) return BPMAX_EINVAL;
+    if (schedule != BPMAX_SCHED_TASKS && schedule != BPMAX_SCHED_LAUNCH && schedule != BPMAX_SCHED_PIPE)
+        return BPMAX_EINVAL;
+    ctx->schedule = schedule;
+    if (chunk > 0) ctx->task_chunk = chunk;
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable)
+{
+    if (!ctx) return BPMAX_EINVAL;
+    ctx->graphs = enable != 0;
     return BPMAX_OK;
 }
 
@@ -270,104 +305,209 @@ static int pick_chunk(long long total_steps, long long tiles)
     return (int)ch;
 }
 
-// Build the launch list for a set of pairs (descs already laid out).
+// Build the per-phase launch list for a set of pairs (descs already laid out), one group
+// per storage tile size T:
+//   S2 by logical block diagonal (cross-tile launch, then closure launch);
+//   for every outer diagonal d1: one R0 launch per storage block diagonal D (all storage
+//   tiles (I,I+D) of every triangle on d1), then the inner closure by logical block
+//   diagonal DL (cross-tile launch, closure launch).
+// Every (d1, D) / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
+// runner can attach its events to each.
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S)
 {
     S.items.clear();
     S.launches.clear();
-    int maxM = 0, maxnb = 0;
-    std::vector<int> Ts;
+    S.groups.clear();
     for (auto &p : P) {
-        maxM = std::max(maxM, p.M);
-        maxnb = std::max(maxnb, p.nb);
-        if (std::find(Ts.begin(), Ts.end(), p.T) == Ts.end()) Ts.push_back(p.T);
+        int g = -1;
+        for (size_t k = 0; k < S.groups.size(); ++k)
+            if (S.groups[k].T == p.T) g = (int)k;
+        if (g < 0) {
+            S.groups.push_back(GroupInfo{p.T, p.nbl / p.nb, 0, 0, 0});
+            g = (int)S.groups.size() - 1;
+        }
+        GroupInfo &G = S.groups[g];
+        G.maxM = std::max(G.maxM, p.M);
+        G.maxnb = std::max(G.maxnb, p.nb);
+        G.maxnbl = std::max(G.maxnbl, p.nbl);
     }
-    auto emit = [&](int kind, int T, bool ss, size_t first, double upd) {
+    auto emit = [&](int kind, int g, bool ss, size_t first, double upd, int d1, int diag) {
         const int n = (int)(S.items.size() - first);
-        if (n > 0) S.launches.push_back(Launch{kind, T, ss, (int)first, n, upd});
+        S.launches.push_back(Launch{kind, S.groups[g].T, ss, (int)first, n, upd, g, d1, diag});
     };
-    // S2 single-strand table, by inner block diagonal
-    for (int D = 0; D < maxnb; ++D)
-        for (int T : Ts) {
+    for (int g = 0; g < (int)S.groups.size(); ++g) {
+        const GroupInfo &G = S.groups[g];
+        const int T = G.T;
+        // S2, by logical block diagonal
+        for (int D = 0; D < G.maxnbl; ++D) {
             if (D > 0) {
                 size_t f = S.items.size();
                 long long tot = 0;
                 for (size_t p = 0; p < P.size(); ++p)
-                    if (P[p].T == T && P[p].nb > D) tot += (long long)(P[p].nb - D) * D;
+                    if (P[p].T == T && P[p].nbl > D) tot += (long long)(P[p].nbl - D) * D;
                 const int ch = pick_chunk(tot, 0);
                 for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].nb <= D) continue;
-                    for (int I = 0; I + D < P[p].nb; ++I)
+                    if (P[p].T != T || P[p].nbl <= D) continue;
+                    for (int I = 0; I + D < P[p].nbl; ++I)
                         chunked(S.items, WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0}, D, ch, false);
                 }
-                emit(BPMAX_K_INNER_EXT, T, true, f, 0);
+                emit(BPMAX_K_INNER_EXT, g, true, f, 0, -1, D);
             }
             size_t f = S.items.size();
             for (size_t p = 0; p < P.size(); ++p) {
-                if (P[p].T != T || P[p].nb <= D) continue;
-                for (int I = 0; I + D < P[p].nb; ++I) S.items.push_back(WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0});
+                if (P[p].T != T || P[p].nbl <= D) continue;
+                for (int I = 0; I + D < P[p].nbl; ++I) S.items.push_back(WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0});
             }
-            emit(BPMAX_K_INNER_INTRA, T, true, f, 0);
+            emit(BPMAX_K_INNER_INTRA, g, true, f, 0, -1, D);
         }
-    // F table, by outer diagonal
-    for (int d1 = 0; d1 < maxM; ++d1) {
-        for (int T : Ts) {
-            size_t f = S.items.size();
-            long long tot = 0;
-            double upd = 0;
-            for (size_t p = 0; p < P.size(); ++p) {
-                if (P[p].T != T || P[p].M <= d1) continue;
-                const long long nb = P[p].nb;
-                long long per_tri = 0;
-                for (long long D = 0; D < nb; ++D) per_tri += (nb - D) * (D + 1);
-                tot += (long long)(P[p].M - d1) * d1 * per_tri;
-                const double n = P[p].N;
-                const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
-                upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
+        // F table, by outer diagonal
+        for (int d1 = 0; d1 < G.maxM; ++d1) {
+            for (int Ds = 0; Ds < G.maxnb; ++Ds) {
+                size_t f = S.items.size();
+                long long tot = 0;
+                double upd = 0;
+                for (size_t p = 0; p < P.size(); ++p) {
+                    if (P[p].T != T || P[p].M <= d1 || P[p].nb <= Ds) continue;
+                    tot += (long long)(P[p].M - d1) * (P[p].nb - Ds) * d1 * (Ds + 1);
+                    if (Ds == 0) {
+                        const double n = P[p].N;
+                        const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
+                        upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
+                    }
+                }
+                const int ch = pick_chunk(tot, 0);
+                for (size_t p = 0; p < P.size(); ++p) {
+                    if (P[p].T != T || P[p].M <= d1 || P[p].nb <= Ds) continue;
+                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                        for (int I = 0; I + Ds < P[p].nb; ++I)
+                            chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + Ds, 0, 0, 0}, d1 * (Ds + 1), ch,
+                                    true);
+                }
+                emit(BPMAX_K_OUTER, g, false, f, upd, d1, Ds);
             }
-            const int ch = pick_chunk(tot, 0);
-            for (size_t p = 0; p < P.size(); ++p) {
-                if (P[p].T != T || P[p].M <= d1) continue;
-                for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                    for (int I = 0; I < P[p].nb; ++I)
-                        for (int J = I; J < P[p].nb; ++J)
-                            chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, J, 0, 0, 0}, d1 * (J - I + 1), ch, true);
-            }
-            emit(BPMAX_K_OUTER, T, false, f, upd);
-        }
-        for (int D = 0; D < maxnb; ++D)
-            for (int T : Ts) {
+            for (int D = 0; D < G.maxnbl; ++D) {
                 if (D > 0) {
                     size_t f = S.items.size();
                     long long tot = 0;
                     for (size_t p = 0; p < P.size(); ++p)
-                        if (P[p].T == T && P[p].M > d1 && P[p].nb > D)
-                            tot += (long long)(P[p].M - d1) * (P[p].nb - D) * 2 * D;
+                        if (P[p].T == T && P[p].M > d1 && P[p].nbl > D)
+                            tot += (long long)(P[p].M - d1) * (P[p].nbl - D) * 2 * D;
                     const int ch = pick_chunk(tot, 0);
                     for (size_t p = 0; p < P.size(); ++p) {
-                        if (P[p].T != T || P[p].M <= d1 || P[p].nb <= D) continue;
+                        if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D) continue;
                         for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                            for (int I = 0; I + D < P[p].nb; ++I)
+                            for (int I = 0; I + D < P[p].nbl; ++I)
                                 chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, 0}, 2 * D, ch, false);
                     }
-                    emit(BPMAX_K_INNER_EXT, T, false, f, 0);
+                    emit(BPMAX_K_INNER_EXT, g, false, f, 0, d1, D);
                 }
                 size_t f = S.items.size();
                 for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].M <= d1 || P[p].nb <= D) continue;
+                    if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D) continue;
                     for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                        for (int I = 0; I + D < P[p].nb; ++I)
+                        for (int I = 0; I + D < P[p].nbl; ++I)
                             S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, 0});
                 }
-                emit(BPMAX_K_INNER_INTRA, T, false, f, 0);
+                emit(BPMAX_K_INNER_INTRA, g, false, f, 0, d1, D);
             }
+        }
     }
+}
+
+// Task list for the dependency-tile scheduler (k_tasks), in a topological order:
+// S1 of every pair; S2 logical tiles by block diagonal (cross-tile chunks before
+// closures); then per outer diagonal d1, walking the logical block diagonals DL of the
+// inner closure: before DL, the R0 chunks of every storage tile whose first logical
+// sub-tile lies on DL; then the cross-tile chunks and closures of DL.  Every dependence
+// of a task (see wait_deps in bpmax_kernels.hip) precedes it in this order.  flags
+// receives each pair's initial counters (pending = contributions still to land).
+void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
+                 std::vector<int> &flags)
+{
+    tasks.clear();
+    int maxM = 0, maxnbl = 0;
+    for (auto &p : P)
+        if (p.T == T) {
+            maxM = std::max(maxM, p.M);
+            maxnbl = std::max(maxnbl, p.nbl);
+        }
+    auto add = [&](int type, int p, int i1, int j1, int I, int J, int sb, int se, int extra) {
+        tasks.push_back(WorkItem{p, i1, j1, I, J, sb, se, (type << WI_TYPE_SHIFT) | extra});
+    };
+    auto sctr = [&](const PairDesc &d, int X, int Y) -> int * {
+        return flags.data() + d.flag + 2 + 2 * utri_ll(X, Y, d.nbl);
+    };
+    auto fctr = [&](const PairDesc &d, int i1, int j1, int X, int Y) -> int * {
+        return flags.data() + d.flag + 2 + 2 * tri_ll(d.nbl) +
+               2 * (utri_ll(i1, j1, d.M) * tri_ll(d.nbl) + utri_ll(X, Y, d.nbl));
+    };
+    for (size_t p = 0; p < P.size(); ++p)
+        if (P[p].T == T) add(TASK_S1, (int)p, 0, 0, 0, 0, 0, 0, 0);
+    for (int D = 0; D < maxnbl; ++D) {
+        for (size_t p = 0; p < P.size(); ++p) {
+            if (P[p].T != T || P[p].nbl <= D || D == 0) continue;
+            for (int I = 0; I + D < P[p].nbl; ++I)
+                for (int sb = 0; sb < D; sb += chunk) {
+                    add(TASK_S2EXT, (int)p, 0, 0, I, I + D, sb, std::min(D, sb + chunk), WI_ATOMIC);
+                    sctr(P[p], I, I + D)[1] += 1;
+                }
+        }
+        for (size_t p = 0; p < P.size(); ++p) {
+            if (P[p].T != T || P[p].nbl <= D) continue;
+            for (int I = 0; I + D < P[p].nbl; ++I) add(TASK_S2INTRA, (int)p, 0, 0, I, I + D, 0, 0, 0);
+        }
+    }
+    for (int d1 = 0; d1 < maxM; ++d1)
+        for (int DL = 0; DL < maxnbl; ++DL) {
+            // R0 chunks of storage tiles (I, I+Ds) whose lowest logical diagonal is DL
+            for (size_t p = 0; p < P.size(); ++p) {
+                const PairDesc &d = P[p];
+                if (d.T != T || d.M <= d1) continue;
+                const int rs = d.nbl / d.nb;
+                for (int Ds = 0; Ds < d.nb; ++Ds) {
+                    if (std::max(0, rs * Ds - (rs - 1)) != DL) continue;
+                    for (int i1 = 0; i1 + d1 < d.M; ++i1)
+                        for (int I = 0; I + Ds < d.nb; ++I) {
+                            const int J = I + Ds, steps = d1 * (Ds + 1);
+                            int n = 0;
+                            if (steps == 0) {
+                                add(TASK_OUTER, (int)p, i1, i1 + d1, I, J, 0, 0, WI_ATOMIC | WI_EPILOGUE);
+                                n = 1;
+                            } else {
+                                for (int sb = 0; sb < steps; sb += chunk, ++n)
+                                    add(TASK_OUTER, (int)p, i1, i1 + d1, I, J, sb, std::min(steps, sb + chunk),
+                                        WI_ATOMIC | (sb == 0 ? WI_EPILOGUE : 0));
+                            }
+                            for (int x = I * rs; x < I * rs + rs; ++x)
+                                for (int y = J * rs; y < J * rs + rs; ++y)
+                                    if (x <= y) fctr(d, i1, i1 + d1, x, y)[1] += n;
+                        }
+                }
+            }
+            if (DL > 0)
+                for (size_t p = 0; p < P.size(); ++p) {
+                    const PairDesc &d = P[p];
+                    if (d.T != T || d.M <= d1 || d.nbl <= DL) continue;
+                    for (int i1 = 0; i1 + d1 < d.M; ++i1)
+                        for (int I = 0; I + DL < d.nbl; ++I)
+                            for (int sb = 0; sb < 2 * DL; sb += chunk) {
+                                add(TASK_EXT, (int)p, i1, i1 + d1, I, I + DL, sb, std::min(2 * DL, sb + chunk), WI_ATOMIC);
+                                fctr(d, i1, i1 + d1, I, I + DL)[1] += 1;
+                            }
+                }
+            for (size_t p = 0; p < P.size(); ++p) {
+                const PairDesc &d = P[p];
+                if (d.T != T || d.M <= d1 || d.nbl <= DL) continue;
+                for (int i1 = 0; i1 + d1 < d.M; ++i1)
+                    for (int I = 0; I + DL < d.nbl; ++I) add(TASK_INTRA, (int)p, i1, i1 + d1, I, I + DL, 0, 0, 0);
+            }
+        }
 }
 
 } // namespace bpx
 
 // Event-bracketed launch helper.
-static int timed(bpmax_ctx *ctx, int kclass, double updates, const std::function<hipError_t()> &fn)
+static int timed(bpmax_ctx *ctx, hipStream_t st, int kclass, double updates, const std::function<hipError_t()> &fn)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -378,7 +518,7 @@ static int timed(bpmax_ctx *ctx, int kclass, double updates, const std::function
         }
         e0 = ctx->ev_pool[ctx->ev_used++];
         e1 = ctx->ev_pool[ctx->ev_used++];
-        HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(e0, st));
     }
     hipError_t e = fn();
     if (e != hipSuccess) {
@@ -386,7 +526,7 @@ static int timed(bpmax_ctx *ctx, int kclass, double updates, const std::function
         return BPMAX_EDEVICE;
     }
     if (ctx->timing) {
-        HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(e1, st));
         ctx->pending.push_back(PendingTiming{kclass, updates, e0, e1});
     }
     return BPMAX_OK;
@@ -409,6 +549,157 @@ int bpx_collect_timing(bpmax_ctx *ctx)
     }
     ctx->pending.clear();
     ctx->ev_used = 0;
+    return BPMAX_OK;
+}
+
+// One phase launch of the schedule on stream st (nothing for an empty position).
+static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDesc *dP, const WorkItem *dI,
+                      float *arena, const int *dcodes, const float *dw)
+{
+    if (L.count == 0) return BPMAX_OK;
+    const WorkItem *items = dI + L.first;
+    switch (L.kind) {
+    case BPMAX_K_OUTER:
+        return timed(ctx, st, L.kind, L.updates,
+                     [&] { return launch_outer(st, L.T, dP, items, L.count, arena, dcodes, dw); });
+    case BPMAX_K_INNER_EXT:
+        return timed(ctx, st, L.kind, L.updates,
+                     [&] { return launch_inner_ext(st, L.T, L.single_strand, dP, items, L.count, arena); });
+    default:
+        return timed(ctx, st, L.kind, L.updates, [&] {
+            return launch_inner_intra(st, L.T, L.single_strand, dP, items, L.count, arena, dcodes, dw);
+        });
+    }
+}
+
+// Dependency events of the pipelined schedule: fill, S2 done, end, then per T group one
+// per (d1, storage diagonal) of R0 work and one per (d1, logical diagonal) of closure work.
+static int ensure_sync_events(bpmax_ctx *ctx, const Schedule &S, std::vector<size_t> &out_base,
+                              std::vector<size_t> &in_base)
+{
+    size_t n = 3;
+    out_base.assign(S.groups.size(), 0);
+    in_base.assign(S.groups.size(), 0);
+    for (size_t g = 0; g < S.groups.size(); ++g) {
+        const GroupInfo &G = S.groups[g];
+        out_base[g] = n;
+        n += (size_t)G.maxM * G.maxnb;
+        in_base[g] = n;
+        n += (size_t)G.maxM * G.maxnbl;
+    }
+    while (ctx->sync_ev.size() < n) {
+        hipEvent_t e;
+        HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->sync_ev.push_back(e);
+    }
+    return BPMAX_OK;
+}
+
+// Pipelined schedule on two streams: sO runs the fill and every R0 launch, sI runs S1, S2
+// and every inner-closure launch.  R0 of (d1, storage diagonal Ds) reads tiles of the
+// triangles on diagonals < d1 up to storage diagonal Ds, i.e. logical diagonal
+// rs*Ds + rs-1 -- it waits for the closure of d1-1 to reach that logical diagonal (for
+// d1 = 0: for S1/S2).  The closure of (d1, logical diagonal DL) needs R0 done on every
+// storage tile that holds a logical tile of diagonal DL, i.e. storage diagonal
+// ceil(DL/rs).  The closure of d1 thus overlaps the R0 launches of d1 (later storage
+// diagonals) and of d1+1 (earlier ones).
+static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const PairDesc *dP, const WorkItem *dI,
+                        float *arena, long long fl, const int *dcodes, const float *dw)
+{
+    hipStream_t sO = ctx->stream, sI = ctx->stream2;
+    std::vector<size_t> ob, ib;
+    int rc;
+    if ((rc = ensure_sync_events(ctx, S, ob, ib)) != BPMAX_OK) return rc;
+    hipEvent_t ev_fill = ctx->sync_ev[0], ev_s2 = ctx->sync_ev[1], ev_end = ctx->sync_ev[2];
+    auto ev_out = [&](int g, int d1, int D) { return ctx->sync_ev[ob[g] + (size_t)d1 * S.groups[g].maxnb + D]; };
+    auto ev_in = [&](int g, int d1, int D) { return ctx->sync_ev[ib[g] + (size_t)d1 * S.groups[g].maxnbl + D]; };
+
+    if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0, [&] { return launch_fill(sO, arena, fl, -INFINITY); })) != BPMAX_OK)
+        return rc;
+    HIPCHK(ctx, hipEventRecord(ev_fill, sO));
+    HIPCHK(ctx, hipStreamWaitEvent(sI, ev_fill, 0));
+    if ((rc = timed(ctx, sI, BPMAX_K_S1, 0, [&] { return launch_s1(sI, dP, npairs, arena, dcodes, dw); })) !=
+        BPMAX_OK)
+        return rc;
+    for (const Launch &L : S.launches)
+        if (L.d1 < 0 && (rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+    HIPCHK(ctx, hipEventRecord(ev_s2, sI));
+
+    int cur_g = -1, cur_d1 = -1, in_wait = -1, out_wait = -1;
+    for (const Launch &L : S.launches) {
+        if (L.d1 < 0) continue;
+        const GroupInfo &G = S.groups[L.group];
+        if (L.group != cur_g || L.d1 != cur_d1) {
+            cur_g = L.group;
+            cur_d1 = L.d1;
+            in_wait = out_wait = -1;
+        }
+        if (L.kind == BPMAX_K_OUTER) {
+            if (L.d1 == 0) {
+                if (in_wait < 0) HIPCHK(ctx, hipStreamWaitEvent(sO, ev_s2, 0));
+                in_wait = G.maxnbl;
+            } else {
+                const int need = std::min(G.rs * L.diag + G.rs - 1, G.maxnbl - 1);
+                if (need > in_wait) {
+                    HIPCHK(ctx, hipStreamWaitEvent(sO, ev_in(L.group, L.d1 - 1, need), 0));
+                    in_wait = need;
+                }
+            }
+            if ((rc = run_launch(ctx, sO, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+            HIPCHK(ctx, hipEventRecord(ev_out(L.group, L.d1, L.diag), sO));
+        } else {
+            const int need = std::min((L.diag + G.rs - 1) / G.rs, G.maxnb - 1);
+            if (need > out_wait) {
+                HIPCHK(ctx, hipStreamWaitEvent(sI, ev_out(L.group, L.d1, need), 0));
+                out_wait = need;
+            }
+            if ((rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+            if (L.kind == BPMAX_K_INNER_INTRA) HIPCHK(ctx, hipEventRecord(ev_in(L.group, L.d1, L.diag), sI));
+        }
+    }
+    HIPCHK(ctx, hipEventRecord(ev_end, sI));
+    HIPCHK(ctx, hipStreamWaitEvent(sO, ev_end, 0));
+    return BPMAX_OK;
+}
+
+// Capture the pipelined schedule into a graph (kept until the shapes or buffers change)
+// and launch it on ctx->stream.
+static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, int npairs, const Schedule &S,
+                             const PairDesc *dP, const WorkItem *dI, float *arena, long long fl, const int *dcodes,
+                             const float *dw)
+{
+    std::vector<long long> gk = key;
+    for (const DevBuf *b : {&ctx->arena, &ctx->meta, &ctx->codes}) {
+        gk.push_back((long long)(uintptr_t)b->p);
+        gk.push_back((long long)b->cap);
+    }
+    gk.push_back(fl);
+    if (!ctx->graph || gk != ctx->graph_key) {
+        if (ctx->graph) HIPCHK(ctx, hipGraphExecDestroy(ctx->graph));
+        ctx->graph = nullptr;
+        ctx->graph_key.clear();
+        std::vector<size_t> ob, ib;
+        int rc;
+        if ((rc = ensure_sync_events(ctx, S, ob, ib)) != BPMAX_OK) return rc;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_pipe(ctx, npairs, S, dP, dI, arena, fl, dcodes, dw);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != BPMAX_OK || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc == BPMAX_OK) ctx->err = std::string("graph capture: ") + hipGetErrorString(ce);
+            return BPMAX_EDEVICE;
+        }
+        const hipError_t ie = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            ctx->graph = nullptr;
+            ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie);
+            return BPMAX_EDEVICE;
+        }
+        ctx->graph_key = gk;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
     return BPMAX_OK;
 }
 
@@ -455,6 +746,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         d.N = N;
         d.T = choose_tile(N);
         d.nb = (N + d.T - 1) / d.T;
+        d.nbl = d.nb * (d.T / inner_tile(d.T));
         d.tstride = tri_ll(d.nb) * d.T * d.T;
         d.F = fl;
         fl += al(tri_ll(M) * d.tstride);
@@ -477,18 +769,57 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         memcpy(codes.data() + P[p].a, seq1s[p], sizeof(int) * Ms[p]);
         for (int x = 0; x < Ns[p]; ++x) codes[P[p].b + x] = seq2s[p][Ns[p] - 1 - x];
     }
-    Schedule &S = ctx->sched;
-    build_schedule(P, S);
+    // dependency counters of the task scheduler
+    long long nflags = 0;
+    for (auto &d : P) {
+        d.flag = nflags;
+        nflags += pair_flag_ints(d.M, d.nbl);
+    }
+    std::vector<int> Ts;
+    for (auto &d : P)
+        if (std::find(Ts.begin(), Ts.end(), d.T) == Ts.end()) Ts.push_back(d.T);
+    std::sort(Ts.begin(), Ts.end());
 
-    // meta buffer: weights | pair descs | items
+    // The schedule is rebuilt only when the shapes or the schedule settings change.
+    std::vector<long long> key = {ctx->schedule, ctx->task_chunk, npairs};
+    for (auto &d : P) {
+        key.push_back(d.M);
+        key.push_back(d.N);
+    }
+    Schedule &S = ctx->sched;
+    if (key != ctx->sched_key) {
+        ctx->sched_key.clear();
+        ctx->items_on_device = nullptr;
+        if (ctx->schedule == BPMAX_SCHED_TASKS) {
+            S.items.clear();
+            S.launches.clear();
+            S.groups.clear();
+            ctx->flags_host.assign((size_t)nflags, 0);
+            std::vector<WorkItem> tl;
+            for (int T : Ts) {
+                build_tasks(P, T, ctx->task_chunk, tl, ctx->flags_host);
+                S.launches.push_back(
+                    Launch{BPMAX_K_TASKS, T, false, (int)S.items.size(), (int)tl.size(), 0, 0, -1, 0});
+                S.items.insert(S.items.end(), tl.begin(), tl.end());
+            }
+        } else {
+            build_schedule(P, S);
+        }
+        ctx->sched_key = key;
+    }
+
+    // meta buffer: weights | pair descs | items (items uploaded once per schedule)
     const size_t wbytes = 256;
     const size_t pbytes = (P.size() * sizeof(PairDesc) + 255) / 256 * 256;
     const size_t ibytes = S.items.size() * sizeof(WorkItem);
+    const size_t meta_cap = ctx->meta.cap;
     if ((rc = bpx_grow(ctx, ctx->meta, wbytes + pbytes + ibytes)) != BPMAX_OK) return rc;
-    ctx->staging.resize(wbytes + pbytes + ibytes);
+    if (ctx->meta.cap != meta_cap) ctx->items_on_device = nullptr;
+    const bool send_items = ctx->items_on_device != ctx->meta.p;
+    ctx->staging.resize(wbytes + pbytes + (send_items ? ibytes : 0));
     memcpy(ctx->staging.data(), wt, sizeof wt);
     memcpy(ctx->staging.data() + wbytes, P.data(), P.size() * sizeof(PairDesc));
-    memcpy(ctx->staging.data() + wbytes + pbytes, S.items.data(), ibytes);
+    if (send_items) memcpy(ctx->staging.data() + wbytes + pbytes, S.items.data(), ibytes);
     char *meta = (char *)ctx->meta.p;
     const float *dw = (const float *)meta;
     const PairDesc *dP = (const PairDesc *)(meta + wbytes);
@@ -499,31 +830,58 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(ctx->meta.p, ctx->staging.data(), ctx->staging.size(), hipMemcpyHostToDevice,
                                ctx->stream));
+    ctx->items_on_device = ctx->meta.p;
     HIPCHK(ctx, hipMemcpyAsync(ctx->codes.p, codes.data(), codes.size() * sizeof(int), hipMemcpyHostToDevice,
                                ctx->stream));
     hipStream_t st = ctx->stream;
-    if ((rc = timed(ctx, BPMAX_K_FILL, 0, [&] { return launch_fill(st, arena, fl, -INFINITY); })) != BPMAX_OK)
-        return rc;
-    if ((rc = timed(ctx, BPMAX_K_S1, 0, [&] { return launch_s1(st, dP, npairs, arena, dcodes, dw); })) != BPMAX_OK)
-        return rc;
-    for (const Launch &L : S.launches) {
-        const WorkItem *items = dI + L.first;
-        switch (L.kind) {
-        case BPMAX_K_OUTER:
-            rc = timed(ctx, L.kind, L.updates,
-                       [&] { return launch_outer(st, L.T, dP, items, L.count, arena, dcodes, dw); });
-            break;
-        case BPMAX_K_INNER_EXT:
-            rc = timed(ctx, L.kind, L.updates,
-                       [&] { return launch_inner_ext(st, L.T, L.single_strand, dP, items, L.count, arena); });
-            break;
-        default:
-            rc = timed(ctx, L.kind, L.updates, [&] {
-                return launch_inner_intra(st, L.T, L.single_strand, dP, items, L.count, arena, dcodes, dw);
+
+    if (ctx->schedule == BPMAX_SCHED_TASKS) {
+        if ((rc = timed(ctx, st, BPMAX_K_FILL, 0, [&] { return launch_fill(st, arena, fl, -INFINITY); })) !=
+            BPMAX_OK)
+            return rc;
+        // [0..7] per T group: queue heads; [8] error word; then the counters
+        const size_t hdr = 64;
+        if ((rc = bpx_grow(ctx, ctx->flags, (hdr + ctx->flags_host.size()) * sizeof(int))) != BPMAX_OK) return rc;
+        ctx->flags_staging.assign(hdr, 0);
+        ctx->flags_staging.insert(ctx->flags_staging.end(), ctx->flags_host.begin(), ctx->flags_host.end());
+        HIPCHK(ctx, hipMemcpyAsync(ctx->flags.p, ctx->flags_staging.data(), ctx->flags_staging.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        int *dflags = (int *)ctx->flags.p;
+        double upd = 0;
+        for (auto &d : P) upd += bpmax_cell_updates(d.M, d.N);
+        for (size_t g = 0; g < S.launches.size(); ++g) {
+            const Launch &L = S.launches[g];
+            rc = timed(ctx, st, BPMAX_K_TASKS, g == 0 ? upd : 0, [&] {
+                return launch_tasks(st, L.T, dP, dI + L.first, L.count, dflags + g, dflags + hdr, dflags + 8, arena,
+                                    dcodes, dw);
             });
-            break;
+            if (rc != BPMAX_OK) return rc;
         }
+        int err = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&err, dflags + 8, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (err) {
+            ctx->err = "task scheduler: a dependency wait timed out (kernel abandoned its tasks)";
+            return BPMAX_EDEVICE;
+        }
+        return bpx_collect_timing(ctx);
+    }
+
+    if (ctx->schedule == BPMAX_SCHED_PIPE) {
+        if (ctx->graphs && !ctx->timing)
+            rc = launch_pipe_graph(ctx, key, npairs, S, dP, dI, arena, fl, dcodes, dw);
+        else
+            rc = enqueue_pipe(ctx, npairs, S, dP, dI, arena, fl, dcodes, dw);
         if (rc != BPMAX_OK) return rc;
+    } else {
+        if ((rc = timed(ctx, st, BPMAX_K_FILL, 0, [&] { return launch_fill(st, arena, fl, -INFINITY); })) !=
+            BPMAX_OK)
+            return rc;
+        if ((rc = timed(ctx, st, BPMAX_K_S1, 0, [&] { return launch_s1(st, dP, npairs, arena, dcodes, dw); })) !=
+            BPMAX_OK)
+            return rc;
+        for (const Launch &L : S.launches)
+            if ((rc = run_launch(ctx, st, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
     }
     // the host vectors (codes, staging) must outlive the async copies
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -568,9 +926,9 @@ extern "C" int bpmax_engine_solve(bpmax_ctx *ctx, const int *seq1, int M, const 
     float *dout = (float *)ctx->out.p;
     const float *arena = (const float *)ctx->arena.p;
     if (layout == BPMAX_LAYOUT_PACKED_TRI_TRI) {
-        rc = timed(ctx, BPMAX_K_CONVERT, 0, [&] { return launch_to_packed(ctx->stream, arena, P, dout); });
+        rc = timed(ctx, ctx->stream, BPMAX_K_CONVERT, 0, [&] { return launch_to_packed(ctx->stream, arena, P, dout); });
     } else {
-        rc = timed(ctx, BPMAX_K_CONVERT, 0, [&] {
+        rc = timed(ctx, ctx->stream, BPMAX_K_CONVERT, 0, [&] {
             hipError_t e = launch_fill(ctx->stream, dout, (long long)(bytes / sizeof(float)), 0.0f);
             if (e != hipSuccess) return e;
             return launch_to_dense(ctx->stream, arena, P, dout);
@@ -642,7 +1000,7 @@ extern "C" int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const
                                ctx->stream));
     const double nn = n;
     const double upd = (double)batch * (nn + 2) * (nn + 1) * nn / 6.0;
-    rc = timed(ctx, BPMAX_K_R0TRI, upd, [&] {
+    rc = timed(ctx, ctx->stream, BPMAX_K_R0TRI, upd, [&] {
         return launch_r0tri(ctx->stream, T, nb, (const int2 *)ctx->r0meta.p, (int)tiles.size(), batch, A_dev, B_dev,
                             C_dev);
     });

@@ -19,13 +19,22 @@ struct Launch {
     int kind;            // bpmax_kernel_class
     int T;
     bool single_strand;  // S2 phases
-    int first, count;    // work-item range
+    int first, count;    // work-item range (count 0: nothing to launch, events only)
     double updates;      // algorithmic cell updates (outer phases)
+    int group;           // T group (pairs sharing a storage tile size)
+    int d1;              // outer diagonal (-1: S2 phases)
+    int diag;            // storage block diagonal (outer) or logical block diagonal (inner)
+};
+
+struct GroupInfo {
+    int T, rs;           // storage tile, logical tiles per storage tile side
+    int maxM, maxnb, maxnbl;
 };
 
 struct Schedule {
     std::vector<WorkItem> items;
     std::vector<Launch> launches;
+    std::vector<GroupInfo> groups;
 };
 
 struct KStat {
@@ -46,6 +55,8 @@ int choose_tile(int N);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S);
+void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
+                 std::vector<int> &flags);
 
 } // namespace bpx
 
@@ -53,10 +64,21 @@ struct bpmax_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    bpx::DevBuf arena, codes, meta, out, scores, r0meta;
+    bpx::DevBuf arena, codes, meta, out, scores, r0meta, flags;
+    std::vector<int> flags_host, flags_staging;
+    int schedule = BPMAX_SCHED_PIPE;
+    int task_chunk = 16;
     std::vector<bpx::PairDesc> pairs_host;
     bpx::Schedule sched;
+    std::vector<long long> sched_key;   // (schedule, chunk, shapes) the schedule was built for
+    void *items_on_device = nullptr;    // meta buffer that already holds sched.items
     std::vector<char> staging;
+    // pipelined schedule: second stream, dependency events, cached graph
+    hipStream_t stream2 = nullptr;
+    std::vector<hipEvent_t> sync_ev;
+    bool graphs = false;
+    hipGraphExec_t graph = nullptr;
+    std::vector<long long> graph_key;
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;

@@ -19,14 +19,19 @@ constexpr int kMaxTile = 64;
 
 struct PairDesc {
     int M, N;          // strand lengths
-    int nb;            // tiles per inner side
-    int T;             // tile edge (all pairs of one launch share it)
+    int nb;            // storage tiles per inner side
+    int T;             // storage tile edge (all pairs of one launch share it)
+    int nbl;           // logical tiles per inner side for the inner phases (edge min(T, 32))
+    int pad_;
     long long F;       // float offset of triangle (0,0)
     long long tstride; // floats per outer triangle = tri(nb)*T*T
     long long S2;      // float offset of the tiled S2 triangle
     long long S1;      // float offset of dense S1
     long long a;       // int offset of seq1 codes
     long long b;       // int offset of reversed, padded seq2 codes
+    long long flag;    // int offset of the pair's dependency counters (task scheduler):
+                       //   [0] S1 done, [1] S2 tiles done, then (done, pending) per S2 tile,
+                       //   then (done, pending) per F tile (triangle-major, tile-minor)
 };
 
 // One output tile of one phase.  Steps index the K-dimension of that tile's max-plus
@@ -42,7 +47,20 @@ struct WorkItem {
 enum : int {
     WI_ATOMIC = 1,     // several items write this tile: combine with atomic max
     WI_EPILOGUE = 2,   // this item also adds init/P1/R3/R4 (outer phase)
+    WI_TYPE_SHIFT = 4, // task type (scheduler) in the bits above
 };
+
+enum : int { TASK_S1 = 1, TASK_S2EXT, TASK_S2INTRA, TASK_OUTER, TASK_EXT, TASK_INTRA };
+
+// counters of one pair: S1 done, S2 done-count, (done, pending) per logical S2 tile and
+// per logical F tile; nbl = logical tiles per inner side
+__host__ __device__ inline long long pair_flag_ints(long long M, long long nbl)
+{
+    return 2 + 2 * (nbl * (nbl + 1) / 2) + 2 * (M * (M + 1) / 2) * (nbl * (nbl + 1) / 2);
+}
+
+// edge of the logical tiles the inner phases run on, for storage tile T
+__host__ __device__ constexpr int inner_tile(int T) { return T < 32 ? T : 32; }
 
 __host__ __device__ inline long long tri_ll(long long n) { return n * (n + 1) / 2; }
 __host__ __device__ inline long long utri_ll(long long i, long long j, long long n)

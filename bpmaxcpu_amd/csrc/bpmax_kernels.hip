@@ -1,160 +1,77 @@
 // bpmax_kernels.hip -- hand-written gfx950 kernels of the BPMax engine.
 //
 // Max-plus ("tropical") products are not multiply-accumulates, so nothing here uses MFMA:
-// the hot loop is VALU v_add_f32 + v_max3_f32 on per-lane 4x4 register tiles fed from
-// LDS with 128-bit reads (1.5 VALU instructions per cell update; packed FP32 ops are
-// disabled at compile time).  Scores are exact multiples of 2^-8 far below 2^24, so every
-// sum is exact and every reduction order gives the same bits.
+// the hot loop is VALU v_add_f32 + v_max3_f32 on per-lane register tiles fed from LDS
+// with vector reads (1.5 VALU instructions per cell update; packed FP32 ops are disabled
+// at compile time).  Scores are exact multiples of 2^-p far below 2^24, so every sum is
+// exact and every reduction order gives the same bits.
 //
-// Phases (reference functions they replace, paths relative to
+// Phase bodies (reference functions they replace, paths relative to
 // TPDS/source/release/MPT_RT_V3/bpmax_complete_otf_ftable/src/):
-//   k_outer       R0 double max-plus + init + P1 + R3 + R4 for one outer diagonal
-//                 (bpmax_outer_reductions.c:123-180, matrix_max_plus_section.c:153-257,
-//                  reg_3_24.c:9-84, bpmax_ftable_init.c:116-215,
-//                  bpmax_outer_south_west.c:115-160, bpmax_r3_section.c:114-139,
-//                  bpmax_r4_section.c:114-139)
-//   k_inner_ext   cross-tile parts of R1/R2 (F) and of S2's split term
-//                 (bpmax_inner_reductions.c:140-178, bpmax_single_strand_s2_tile.c:124-262)
-//   k_inner_intra in-tile dependence wavefront with P2 (bpmax_inner_reductions_diagonal_tile.c,
-//                 bpmax_inner_reductions_finalize.c, bpmax_single_strand_diagonal_tile.c,
-//                 bpmax_single_strand_finalize.c)
-//   k_s1          strand-1 single-strand table (bpmax_single_strand_s1.c:115-166)
+//   outer_body  R0 double max-plus + init + P1 + R3 + R4 of one 64 x 64 storage tile
+//               (bpmax_outer_reductions.c:123-180, matrix_max_plus_section.c:153-257,
+//                reg_3_24.c:9-84, bpmax_ftable_init.c:116-215,
+//                bpmax_outer_south_west.c:115-160, bpmax_r3_section.c:114-139,
+//                bpmax_r4_section.c:114-139)
+//   ext_body    cross-tile parts of R1/R2 (F) and of S2's split term on 32 x 32 logical
+//               tiles (bpmax_inner_reductions.c:140-178, bpmax_single_strand_s2_tile.c:124-262)
+//   intra_body  in-tile dependence wavefront with P2 on 32 x 32 logical tiles
+//               (bpmax_inner_reductions_diagonal_tile.c, bpmax_inner_reductions_finalize.c,
+//                bpmax_single_strand_diagonal_tile.c, bpmax_single_strand_finalize.c)
+//   s1_body     strand-1 single-strand table (bpmax_single_strand_s1.c:115-166)
+// They run either as one launch per phase and diagonal (k_outer / k_inner_ext /
+// k_inner_intra) or as tasks of the dependency-tile scheduler k_tasks, which replaces the
+// reference's per-diagonal fork/join (src/bpmax.c:283-324) by per-tile ready counters.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "bpmax_internal.h"
 #include "bpmax_launch.h"
+#include "bpmax_tiles.h"
 
 namespace bpx {
 
-#define NEG_INF (-__builtin_inff())
-
+// logical tile edge of the inner phases for storage tile T
 template <int T>
-struct Tc {
-    static constexpr int R = 4;        // register tile edge: each lane owns R x R cells
-    static constexpr int TX = T / R;   // lanes per tile row
-    static constexpr int NT = TX * TX; // lanes per product workgroup (256 for T=64)
-    static constexpr int LD = T + 4;   // LDS row stride in floats (keeps 16-byte rows)
+struct Inner {
+    static constexpr int LT = T < 32 ? T : 32;
+    static constexpr int RS = T / LT;
 };
 
-__device__ __forceinline__ const float *tile_at(const float *tri, int I, int J, int nb, int T)
-{
-    return tri + utri_ll(I, J, nb) * (long long)(T * T);
-}
-__device__ __forceinline__ float *tile_at(float *tri, int I, int J, int nb, int T)
-{
-    return tri + utri_ll(I, J, nb) * (long long)(T * T);
-}
-
-// As[k][r] = X[I*T + r][K*T + k + cshift]  (cshift 0 or -1); -inf where the source tile
-// lies below the diagonal.  With lastcol_only, every column but k = T-1 reads -inf.
-template <int T>
-__device__ __forceinline__ void load_A(float *As, const float *X, int nb, int I, int K, int cshift,
-                                       bool lastcol_only)
-{
-    constexpr int NT = Tc<T>::NT, LD = Tc<T>::LD;
-    for (int e = threadIdx.x; e < T * T; e += NT) {
-        const int r = e / T, k = e % T;
-        const int y = K * T + k + cshift;
-        float v = NEG_INF;
-        if (y >= 0 && !(lastcol_only && k != T - 1)) {
-            const int Y = y / T;
-            if (Y >= I && Y < nb) v = tile_at(X, I, Y, nb, T)[r * T + (y % T)];
-        }
-        As[k * LD + r] = v;
-    }
-}
-
-// Bs[k][c] = X[K*T + k + rshift][J*T + c]  (rshift 0 or +1); -inf rows below the diagonal.
-template <int T>
-__device__ __forceinline__ void load_B(float *Bs, const float *X, int nb, int K, int J, int rshift)
-{
-    constexpr int NT = Tc<T>::NT, LD = Tc<T>::LD;
-    constexpr int Q = T / 4;
-    for (int e = threadIdx.x; e < T * Q; e += NT) {
-        const int k = e / Q, c4 = (e % Q) * 4;
-        const int x = K * T + k + rshift;
-        const int Xt = x / T;
-        float4 v = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
-        if (Xt <= J && Xt < nb) v = *(const float4 *)(tile_at(X, Xt, J, nb, T) + (x % T) * T + c4);
-        *(float4 *)(Bs + k * LD + c4) = v;
-    }
-}
-
-// acc[i][j] = max(acc[i][j], max_k As[k][ty*4+i] + Bs[k][tx*4+j]) over the whole tile depth.
-template <int T>
-__device__ __forceinline__ void product(float (&acc)[4][4], const float *As, const float *Bs, int ty,
-                                        int tx)
-{
-    constexpr int LD = Tc<T>::LD;
-#pragma unroll 4
-    for (int k = 0; k < T; k += 2) {
-        const float4 a0 = *(const float4 *)(As + k * LD + ty * 4);
-        const float4 a1 = *(const float4 *)(As + (k + 1) * LD + ty * 4);
-        const float4 b0 = *(const float4 *)(Bs + k * LD + tx * 4);
-        const float4 b1 = *(const float4 *)(Bs + (k + 1) * LD + tx * 4);
-        const float av0[4] = {a0.x, a0.y, a0.z, a0.w}, av1[4] = {a1.x, a1.y, a1.z, a1.w};
-        const float bv0[4] = {b0.x, b0.y, b0.z, b0.w}, bv1[4] = {b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = fmaxf(acc[i][j], fmaxf(av0[i] + bv0[j], av1[i] + bv1[j]));
-    }
-}
-
-// Write the lane's 4x4 cells of tile `dst`; atomic = max-combine with what is there
-// (valid because every defined cell's final value is >= 0, where float order equals the
-// signed-integer order of the bit patterns).
-template <int T>
-__device__ __forceinline__ void store_acc(float *dst, const float (&acc)[4][4], int ty, int tx, bool atomic)
-{
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float *row = dst + (ty * 4 + i) * T + tx * 4;
-        if (atomic) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) atomicMax((int *)(row + j), __float_as_int(acc[i][j]));
-        } else {
-            *(float4 *)row = make_float4(acc[i][0], acc[i][1], acc[i][2], acc[i][3]);
-        }
-    }
-}
-
 // ------------------------------------------------------------------------------ outer
-// One work item = one output tile (I,J) of triangle (i1,j1) on the current outer diagonal
-// and a range of K-steps; step s <-> (k1 = i1 + s/(J-I+1), K = I + s%(J-I+1)):
+// Output storage tile (I,J) of triangle (i1,j1) over K-steps [step_begin, step_end):
+// step s <-> (k1 = i1 + s/(J-I+1), K = I + s%(J-I+1)):
 //   C[I][J] max= F(i1,k1)[I][K] (x) F(k1+1,j1)[K*T+1 .. K*T+T][J]        (R0, k2+1 shift)
-// Items flagged WI_EPILOGUE add S1(i1,j1)+S2 (or the inter base case on d1 = 0), P1 and
-// the R3/R4 streams of the same tile.
+// With WI_EPILOGUE also S1(i1,j1)+S2 (the inter base case on the diagonal when d1 = 0),
+// P1 and the R3/R4 streams of the same tile.
 template <int T>
-__global__ __launch_bounds__(Tc<T>::NT) void k_outer(const PairDesc *__restrict__ pairs,
-                                                     const WorkItem *__restrict__ items,
-                                                     float *__restrict__ arena,
-                                                     const int *__restrict__ codes,
-                                                     const float *__restrict__ w)
+__device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
+                           const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
-    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD;
-    __shared__ __attribute__((aligned(16))) float As[T * LD];
-    __shared__ __attribute__((aligned(16))) float Bs[T * LD];
-    const WorkItem it = items[blockIdx.x];
-    const PairDesc P = pairs[it.pair];
+    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD, R = Tc<T>::R;
+    float *As = smem, *Bs = smem + T * LD;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int i1 = it.i1, j1 = it.j1, I = it.I, J = it.J, nb = P.nb, M = P.M;
-    float acc[4][4];
+    float acc[R][R];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = NEG_INF;
+        for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
 
     const int nK = J - I + 1;
-    for (int s = it.step_begin; s < it.step_end; ++s) {
+    FetchA<T> fa;
+    FetchB<T> fb;
+    auto fetch = [&](int s) {
         const int k1 = i1 + s / nK, K = I + s % nK;
-        const float *A = arena + P.F + utri_ll(i1, k1, M) * P.tstride;
-        const float *B = arena + P.F + utri_ll(k1 + 1, j1, M) * P.tstride;
+        fetch_A<T, T>(fa, tri_at(arena, P, i1, k1), nb, I, K, 0);
+        fetch_B<T, T>(fb, tri_at(arena, P, k1 + 1, j1), nb, K, J, 1);
+    };
+    if (it.step_begin < it.step_end) fetch(it.step_begin);
+    for (int s = it.step_begin; s < it.step_end; ++s) {
         __syncthreads();
-        load_A<T>(As, A, nb, I, K, 0, false);
-        load_B<T>(Bs, B, nb, K, J, 1);
+        commit_A<T>(As, fa, 0, false);
+        commit_B<T>(Bs, fb);
         __syncthreads();
+        if (s + 1 < it.step_end) fetch(s + 1); // in flight while this step computes
         product<T>(acc, As, Bs, ty, tx);
     }
 
@@ -165,208 +82,282 @@ __global__ __launch_bounds__(Tc<T>::NT) void k_outer(const PairDesc *__restrict_
         const int *a = codes + P.a, *b = codes + P.b;
         const float s1 = S1[i1 * M + j1];
         const int d1 = j1 - i1;
-        float v[4][4];
+        const float w1 = d1 >= 4 ? w[5 * a[i1] + a[j1]] : 0.f;
+        const float *Pt = d1 >= 4 ? tri_at(arena, P, i1 + 1, j1 - 1) + toff : S2t;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < R; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[i][j] = s1 + S2t[(ty * 4 + i) * T + tx * 4 + j];
-        if (d1 == 0) {
+            for (int j = 0; j < R; ++j) {
+                const int lr = ty * R + i, lc = tx * R + j;
+                float v = s1 + S2t[lr * T + lc];
+                if (d1 == 0 && I * T + lr == J * T + lc) v = w[25 + 5 * a[i1] + b[I * T + lr]]; // inter base
+                if (d1 >= 4) v = fmaxf(v, Pt[lr * T + lc] + w1);
+                acc[i][j] = fmaxf(acc[i][j], v);
+            }
+        // R3 / R4 streams over k1, with the next k1's tiles in flight
+        float ra[R][R], rb[R][R];
+        auto load = [&](int k1) {
+            const float *Bt = tri_at(arena, P, k1 + 1, j1) + toff;
+            const float *At = tri_at(arena, P, i1, k1) + toff;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < R; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int r = I * T + ty * 4 + i, c = J * T + tx * 4 + j;
-                    if (r == c) v[i][j] = w[25 + 5 * a[i1] + b[r]]; // inter(a[i1], b[i2])
+                for (int j = 0; j < R; ++j) {
+                    const int o = (ty * R + i) * T + tx * R + j;
+                    rb[i][j] = Bt[o];
+                    ra[i][j] = At[o];
                 }
-        }
-        if (d1 >= 4) {
-            const float w1 = w[5 * a[i1] + a[j1]];
-            const float *Pt = arena + P.F + utri_ll(i1 + 1, j1 - 1, M) * P.tstride + toff;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) v[i][j] = fmaxf(v[i][j], Pt[(ty * 4 + i) * T + tx * 4 + j] + w1);
-        }
+        };
+        if (i1 < j1) load(i1);
         for (int k1 = i1; k1 < j1; ++k1) {
             const float s1a = S1[i1 * M + k1], s1b = S1[(k1 + 1) * M + j1];
-            const float *Bt = arena + P.F + utri_ll(k1 + 1, j1, M) * P.tstride + toff;
-            const float *At = arena + P.F + utri_ll(i1, k1, M) * P.tstride + toff;
+            float ca[R][R], cb[R][R];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float4 bb = *(const float4 *)(Bt + (ty * 4 + i) * T + tx * 4);
-                const float4 aa = *(const float4 *)(At + (ty * 4 + i) * T + tx * 4);
-                v[i][0] = fmaxf(v[i][0], fmaxf(s1a + bb.x, aa.x + s1b));
-                v[i][1] = fmaxf(v[i][1], fmaxf(s1a + bb.y, aa.y + s1b));
-                v[i][2] = fmaxf(v[i][2], fmaxf(s1a + bb.z, aa.z + s1b));
-                v[i][3] = fmaxf(v[i][3], fmaxf(s1a + bb.w, aa.w + s1b));
-            }
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    ca[i][j] = ra[i][j];
+                    cb[i][j] = rb[i][j];
+                }
+            if (k1 + 1 < j1) load(k1 + 1);
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], fmaxf(s1a + cb[i][j], ca[i][j] + s1b));
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = fmaxf(acc[i][j], v[i][j]);
     }
-    float *C = arena + P.F + utri_ll(i1, j1, M) * P.tstride + toff;
-    store_acc<T>(C, acc, ty, tx, (it.flags & WI_ATOMIC) != 0);
+    store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, ty, tx, (it.flags & WI_ATOMIC) != 0);
 }
 
 // ------------------------------------------------------------------------ inner ext
-// Cross-tile part of the inner closure for tile (I,J), D = J-I >= 1, of X = F(i1,j1)
-// (MODE_S = false) or X = S2 (MODE_S = true), S = S2.  Steps:
-//   F: s <  D : R1  X[I][K] (x) S[K*T+1 ..][J]          K = I+s      (k2 in tile K)
-//      s >= D : R2  S[I][K'*T-1 ..] (x) X[K'][J]        K' = I+1+s-D (k2+1 in tile K')
-//   S: s <  D : S[I][K] (x) S[K*T+1 ..][J], K = I+s; for K = I only column T-1 of S[I][I]
+// Cross-tile part of the inner closure for logical tile (I,J), D = J-I >= 1, of
+// X = F(i1,j1) (MODE_S = false) or X = S2 (MODE_S = true), S = S2.  Steps:
+//   F: s <  D : R1  X[I][K] (x) S[K*LT+1 ..][J]          K = I+s      (k2 in tile K)
+//      s >= D : R2  S[I][K'*LT-1 ..] (x) X[K'][J]        K' = I+1+s-D (k2+1 in tile K')
+//   S: s <  D : S[I][K] (x) S[K*LT+1 ..][J], K = I+s; for K = I only column LT-1 of S[I][I]
 //               (the remaining columns of tile I pair with cells of (I,J) itself and are
 //               handled by the in-tile wavefront).
+// Always max-combined into the tile (other items of the same tile run concurrently).
 template <int T, bool MODE_S>
-__global__ __launch_bounds__(Tc<T>::NT) void k_inner_ext(const PairDesc *__restrict__ pairs,
-                                                         const WorkItem *__restrict__ items,
-                                                         float *__restrict__ arena)
+__device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
 {
-    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD;
-    __shared__ __attribute__((aligned(16))) float As[T * LD];
-    __shared__ __attribute__((aligned(16))) float Bs[T * LD];
-    const WorkItem it = items[blockIdx.x];
-    const PairDesc P = pairs[it.pair];
+    constexpr int LT = Inner<T>::LT;
+    constexpr int TX = Tc<LT>::TX, LD = Tc<LT>::LD, R = Tc<LT>::R;
+    float *As = smem, *Bs = smem + LT * LD;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int I = it.I, J = it.J, nb = P.nb, D = J - I;
     float *S = arena + P.S2;
-    float *X = MODE_S ? S : arena + P.F + utri_ll(it.i1, it.j1, P.M) * P.tstride;
-    float acc[4][4];
+    float *X = MODE_S ? S : tri_at(arena, P, it.i1, it.j1);
+    float acc[R][R];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = NEG_INF;
-    for (int s = it.step_begin; s < it.step_end; ++s) {
-        __syncthreads();
+        for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
+    FetchA<LT> fa;
+    FetchB<LT> fb;
+    int cshift = 0;
+    bool lastcol = false;
+    auto fetch = [&](int s) {
         if (MODE_S) {
             const int K = I + s;
-            load_A<T>(As, S, nb, I, K, 0, K == I);
-            load_B<T>(Bs, S, nb, K, J, 1);
+            cshift = 0;
+            lastcol = K == I;
+            fetch_A<T, LT>(fa, S, nb, I, K, 0);
+            fetch_B<T, LT>(fb, S, nb, K, J, 1);
         } else if (s < D) {
             const int K = I + s;
-            load_A<T>(As, X, nb, I, K, 0, false);
-            load_B<T>(Bs, S, nb, K, J, 1);
+            cshift = 0;
+            lastcol = false;
+            fetch_A<T, LT>(fa, X, nb, I, K, 0);
+            fetch_B<T, LT>(fb, S, nb, K, J, 1);
         } else {
             const int K2 = I + 1 + (s - D);
-            load_A<T>(As, S, nb, I, K2, -1, false);
-            load_B<T>(Bs, X, nb, K2, J, 0);
+            cshift = -1;
+            lastcol = false;
+            fetch_A<T, LT>(fa, S, nb, I, K2, -1);
+            fetch_B<T, LT>(fb, X, nb, K2, J, 0);
         }
+    };
+    if (it.step_begin < it.step_end) fetch(it.step_begin);
+    for (int s = it.step_begin; s < it.step_end; ++s) {
         __syncthreads();
-        product<T>(acc, As, Bs, ty, tx);
+        commit_A<LT>(As, fa, cshift, lastcol);
+        commit_B<LT>(Bs, fb);
+        __syncthreads();
+        if (s + 1 < it.step_end) fetch(s + 1);
+        product<LT>(acc, As, Bs, ty, tx);
     }
-    store_acc<T>(tile_at(X, I, J, nb, T), acc, ty, tx, true);
+    store_acc<T, LT>(ltile<T, LT>(X, I, J, nb), acc, ty, tx, true);
 }
 
 // ---------------------------------------------------------------------- inner intra
-// In-tile closure of tile (I,J) of X (F(i1,j1) or S2): processes the cells by
-// anti-diagonal step s = b + (T-1-a) (local row a, column b); every dependence of a cell
+// In-tile closure of logical tile (I,J) of X (F(i1,j1) or S2): cells are processed by
+// anti-diagonal step s = b + (LT-1-a) (local row a, column b); every dependence of a cell
 // (same row to the left, same column below, P2 at (a+1,b-1)) lies on an earlier step.
-// For each cell, G lanes split the in-tile terms
+// G = 256/LT lanes share a cell and split its in-tile terms
 //   R1: X[a][b'] + SR[b'+1][b]   b' in [diag ? a : 0, b-1]   SR = S2 tile (J,J)
-//   R2: SL[a][a'-1] + X[a'][b]   a' in [a+1, diag ? b : T-1] SL = S2 tile (I,I)
-// and combine by lane shuffles; then P2 = X[a+1][b-1] + intra(b[r], b[c]) when c-r >= 4.
-// For S2 on a diagonal tile SL = SR = X itself (both factors of S2's split term are in
-// the tile); S2 additionally starts every upper cell at 0 (its short-interval value).
+//   R2: SL[a][a'-1] + X[a'][b]   a' in [a+1, diag ? b : LT-1] SL = S2 tile (I,I)
+// with U = 32/G independent accumulators (U LDS reads in flight per round trip) and no
+// per-term guard: the last round of a lane may run up to PAD = U*G terms past its range,
+// and every such term is -inf by construction -- past the diagonal the S2 factor is a
+// lower-triangle (-inf) entry, past the tile edge it reads the -inf pad rows of Xs / Sb.
+// Lanes then combine with DPP permutes and the cell's P2 = X[a+1][b-1] + intra(b[r], b[c])
+// (when c-r >= 4, fetched before the reduction).  For S2 on a diagonal tile SL = SR = X
+// itself (both factors of S2's split term lie in the tile); S2 additionally starts every
+// upper cell at 0 (its short-interval value).
 template <int T, bool MODE_S>
-__global__ __launch_bounds__(256) void k_inner_intra(const PairDesc *__restrict__ pairs,
-                                                     const WorkItem *__restrict__ items,
-                                                     float *__restrict__ arena,
-                                                     const int *__restrict__ codes,
-                                                     const float *__restrict__ w)
+__device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
+                           const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
-    constexpr int NTI = 256, G = NTI / T, L = T + 1;
-    __shared__ float Xs[T * L], Sa[T * L], Sb[T * L];
-    __shared__ float below[T], left[T];
-    __shared__ float corner;
-    __shared__ int cr[T], cc[T];
-    const WorkItem it = items[blockIdx.x];
-    const PairDesc P = pairs[it.pair];
-    const int I = it.I, J = it.J, nb = P.nb;
+    constexpr int LT = Inner<T>::LT;
+    using C = Intra<LT>;
+    constexpr int G = C::G, U = C::U, PAD = C::PAD, L = C::L;
+    float *Xs = smem + C::XS, *Sa = smem + C::SA, *Sb = smem + C::SB;
+    float *below = smem + C::BELOW, *left = below + LT, *cornerp = left + LT;
+    float *wt = smem + C::WT, *wl = smem + C::WL;
+    int *cr = (int *)(smem + C::CODES), *cc = cr + LT;
+    const int nb = P.nb;
     const bool diag = (I == J);
     float *S = arena + P.S2;
-    float *X = MODE_S ? S : arena + P.F + utri_ll(it.i1, it.j1, P.M) * P.tstride;
+    float *X = MODE_S ? S : tri_at(arena, P, i1, j1);
     const int tid = threadIdx.x;
-    const float *Xt = tile_at(X, I, J, nb, T);
-    for (int e = tid; e < T * T; e += NTI) {
-        const int a = e / T, b = e % T;
-        float v = Xt[e];
-        if (MODE_S && I * T + a <= J * T + b) v = fmaxf(v, 0.0f);
-        Xs[a * L + b] = v;
+
+    // tiles: every global load issued before any LDS store (float4, unrolled, branch-free)
+    constexpr int Q = LT / 4, PER = LT * Q / kThreads;
+    const float *Xt = ltile<T, LT>(X, I, J, nb);
+    const float *SLt = ltile<T, LT>(S, I, I, nb), *SRt = ltile<T, LT>(S, J, J, nb);
+    float4 vx[PER], va[PER], vb[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = tid + p * kThreads, a = idx / Q, c4 = (idx % Q) * 4;
+        vx[p] = *(const float4 *)(Xt + a * T + c4);
+        va[p] = *(const float4 *)(SLt + a * T + c4);
+        vb[p] = *(const float4 *)(SRt + a * T + c4);
     }
-    const float *SL = Sa, *SR = Sb;
-    if (diag) {
-        if (MODE_S) {
-            SL = Xs;
-            SR = Xs;
-        } else {
-            const float *St = tile_at(S, I, I, nb, T);
-            for (int e = tid; e < T * T; e += NTI) Sa[(e / T) * L + e % T] = St[e];
-            SR = Sa;
-        }
-    } else {
-        const float *S1t = tile_at(S, I, I, nb, T), *S2t = tile_at(S, J, J, nb, T);
-        for (int e = tid; e < T * T; e += NTI) {
-            Sa[(e / T) * L + e % T] = S1t[e];
-            Sb[(e / T) * L + e % T] = S2t[e];
-        }
-        const float *Bt = tile_at(X, I + 1, J, nb, T);
-        const float *Lt = tile_at(X, I, J - 1, nb, T);
-        for (int e = tid; e < T; e += NTI) {
-            below[e] = Bt[e];
-            left[e] = Lt[e * T + T - 1];
-        }
-        if (tid == 0) corner = (J - I >= 2) ? tile_at(X, I + 1, J - 1, nb, T)[T - 1] : NEG_INF;
+    float bl = NEG_INF, lf = NEG_INF, cn = NEG_INF;
+    if (!diag && tid < LT) {
+        bl = ltile<T, LT>(X, I + 1, J, nb)[tid];
+        lf = ltile<T, LT>(X, I, J - 1, nb)[tid * T + LT - 1];
+        if (tid == 0 && J - I >= 2) cn = ltile<T, LT>(X, I + 1, J - 1, nb)[LT - 1];
     }
     const int *bc = codes + P.b;
-    for (int e = tid; e < T; e += NTI) {
-        cr[e] = bc[I * T + e];
-        cc[e] = bc[J * T + e];
+    int vcr = 0, vcc = 0;
+    if (tid < LT) {
+        vcr = bc[I * LT + tid];
+        vcc = bc[J * LT + tid];
     }
+    const float vw = tid < 25 ? w[tid] : 0.f;
+    // -inf pads: columns LT..L-1 of every row, rows LT..LT+PAD-1 of Xs and Sb
+    for (int e = tid; e < (LT + PAD) * (L - LT); e += kThreads) {
+        const int a = e / (L - LT), b = LT + e % (L - LT);
+        Xs[a * L + b] = NEG_INF;
+        Sb[a * L + b] = NEG_INF;
+        if (a < LT) Sa[a * L + b] = NEG_INF;
+    }
+    for (int e = tid; e < PAD * LT; e += kThreads) {
+        const int a = LT + e / LT, b = e % LT;
+        Xs[a * L + b] = NEG_INF;
+        Sb[a * L + b] = NEG_INF;
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = tid + p * kThreads, a = idx / Q, c4 = (idx % Q) * 4;
+        float4 x = vx[p];
+        if (MODE_S) { // S2 starts every upper cell at 0, its short-interval value
+            const int r = I * LT + a, c = J * LT + c4;
+            if (r <= c) x.x = fmaxf(x.x, 0.f);
+            if (r <= c + 1) x.y = fmaxf(x.y, 0.f);
+            if (r <= c + 2) x.z = fmaxf(x.z, 0.f);
+            if (r <= c + 3) x.w = fmaxf(x.w, 0.f);
+        }
+        *(float4 *)(Xs + a * L + c4) = x;
+        *(float4 *)(Sa + a * L + c4) = va[p];               // S2 tile (I,I)
+        *(float4 *)(Sb + a * L + c4) = diag ? va[p] : vb[p]; // S2 tile (J,J)
+    }
+    if (tid < LT) {
+        below[tid] = bl;
+        left[tid] = lf;
+        cr[tid] = vcr;
+        cc[tid] = vcc;
+    }
+    if (tid == 0) cornerp[0] = cn;
+    if (tid < 25) wl[tid] = vw;
+    // SL = S2 (I,I), SR = S2 (J,J) (both in Sb on a diagonal tile); S2's own diagonal tiles
+    // take both factors from the tile being computed
+    const float *SL = diag ? (MODE_S ? Xs : Sb) : Sa;
+    const float *SR = diag && MODE_S ? Xs : Sb;
+    __syncthreads();
+    // P2 weights of every cell: wt[a][b] = intra(b[I*LT+a], b[J*LT+b])
+    for (int e = tid; e < LT * LT; e += kThreads) wt[e] = wl[5 * cr[e / LT] + cc[e % LT]];
     __syncthreads();
 
     const int q = tid / G, g = tid % G;
-    for (int s = diag ? T : 0; s <= 2 * T - 2; ++s) {
-        const int alo = s < T - 1 ? T - 1 - s : 0;
-        const int ahi = s > T - 1 ? 2 * T - 2 - s : T - 1;
+    for (int s = diag ? LT : 0; s <= 2 * LT - 2; ++s) {
+        const int alo = s < LT - 1 ? LT - 1 - s : 0;
+        const int ahi = s > LT - 1 ? 2 * LT - 2 - s : LT - 1;
         const int a = alo + q;
         const bool live = a <= ahi;
-        const int b = a + s - (T - 1);
-        float v = NEG_INF;
-        if (live) {
-            const int lo1 = diag ? a : 0;
-            for (int bp = lo1 + g; bp < b; bp += G) v = fmaxf(v, Xs[a * L + bp] + SR[(bp + 1) * L + b]);
-            const int hi2 = diag ? b : T - 1;
-            for (int ap = a + 1 + g; ap <= hi2; ap += G) v = fmaxf(v, SL[a * L + ap - 1] + Xs[ap * L + b]);
-        }
+        const int b = a + s - (LT - 1);
+        float vv[U];
 #pragma unroll
-        for (int off = 1; off < G; off <<= 1) v = fmaxf(v, __shfl_xor(v, off));
-        if (live && g == 0) {
-            v = fmaxf(v, Xs[a * L + b]);
-            const int r = I * T + a, c = J * T + b;
+        for (int u = 0; u < U; ++u) vv[u] = NEG_INF;
+        float cur = NEG_INF; // the cell's own value and its P2 candidate, fetched up front
+        if (live) {
+            const int r = I * LT + a, c = J * LT + b;
+            cur = Xs[a * L + b];
             if (c - r >= 4) {
                 float p;
-                if (a + 1 < T && b >= 1) p = Xs[(a + 1) * L + b - 1];
+                if (a + 1 < LT && b >= 1) p = Xs[(a + 1) * L + b - 1];
                 else if (b >= 1) p = below[b - 1];
-                else if (a + 1 < T) p = left[a + 1];
-                else p = corner;
-                v = fmaxf(v, p + w[5 * cr[a] + cc[b]]);
+                else if (a + 1 < LT) p = left[a + 1];
+                else p = cornerp[0];
+                cur = fmaxf(cur, p + wt[a * LT + b]);
             }
-            Xs[a * L + b] = v;
+            // R1 terms t in [0, n1): X[a][lo1+t] + SR[lo1+t+1][b]; R2 terms t in [0, n2):
+            // SL[a][a+t] + X[a+1+t][b].  Lane g takes t = g, g+G, ...; both streams advance
+            // together, and a stream past its range reads a -inf pad row instead.
+            const int lo1 = diag ? a : 0;
+            const int n1 = b - lo1;
+            const int n2 = (diag ? b : LT - 1) - a;
+            const float *r1x = Xs + a * L + lo1 + g;
+            const float *r1s = SR + (lo1 + 1 + g) * L + b;
+            const float *r2s = SL + a * L + a + g;
+            const float *r2x = Xs + (a + 1 + g) * L + b;
+            const float *r1s_pad = SR + LT * L + b, *r2x_pad = Xs + LT * L + b;
+            const int nmax = n1 > n2 ? n1 : n2;
+            for (int t = g; t < nmax; t += PAD) {
+                const float *ps = t < n1 ? r1s : r1s_pad;
+                const float *px = t < n2 ? r2x : r2x_pad;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    vv[u] = fmaxf(vv[u], fmaxf(r1x[u * G] + ps[u * G * L], r2s[u * G] + px[u * G * L]));
+                r1x += PAD;
+                r1s += PAD * L;
+                r2s += PAD;
+                r2x += PAD * L;
+            }
         }
+        float v = vv[0];
+#pragma unroll
+        for (int u = 1; u < U; ++u) v = fmaxf(v, vv[u]);
+        v = lane_group_max<G>(v);
+        if (live && g == 0) Xs[a * L + b] = fmaxf(v, cur);
         __syncthreads();
     }
-    float *Xo = tile_at(X, I, J, nb, T);
-    for (int e = tid; e < T * T; e += NTI) Xo[e] = Xs[(e / T) * L + e % T];
+    float *Xo = ltile<T, LT>(X, I, J, nb);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = tid + p * kThreads, a = idx / Q, c4 = (idx % Q) * 4;
+        *(float4 *)(Xo + a * T + c4) = *(const float4 *)(Xs + a * L + c4);
+    }
 }
 
 // ------------------------------------------------------------------------------ S1
-// Strand-1 single-strand table, dense M x M, one workgroup per pair, diagonal by diagonal
-// (bpmax_verify.ab:17-21).
-__global__ __launch_bounds__(256) void k_s1(const PairDesc *__restrict__ pairs, float *__restrict__ arena,
-                                            const int *__restrict__ codes, const float *__restrict__ w)
+// Strand-1 single-strand table, dense M x M, one workgroup, diagonal by diagonal
+// (compilation_scripts/bpmax_verify.ab:17-21).
+__device__ void s1_body(const PairDesc &P, float *__restrict__ arena, const int *__restrict__ codes,
+                        const float *__restrict__ w)
 {
-    const PairDesc P = pairs[blockIdx.x];
     float *S1 = arena + P.S1;
     const int *a = codes + P.a;
     const int M = P.M;
@@ -384,6 +375,235 @@ __global__ __launch_bounds__(256) void k_s1(const PairDesc *__restrict__ pairs, 
         }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------- one launch per phase kernels
+template <int T>
+__global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__ pairs,
+                                                    const WorkItem *__restrict__ items, float *__restrict__ arena,
+                                                    const int *__restrict__ codes, const float *__restrict__ w)
+{
+    __shared__ __attribute__((aligned(16))) float smem[Tc<T>::PROD_FLOATS];
+    const WorkItem it = items[blockIdx.x];
+    outer_body<T>(pairs[it.pair], it, arena, codes, w, smem);
+}
+
+template <int T, bool MODE_S>
+__global__ __launch_bounds__(kThreads) void k_inner_ext(const PairDesc *__restrict__ pairs,
+                                                        const WorkItem *__restrict__ items, float *__restrict__ arena)
+{
+    __shared__ __attribute__((aligned(16))) float smem[Tc<Inner<T>::LT>::PROD_FLOATS];
+    const WorkItem it = items[blockIdx.x];
+    ext_body<T, MODE_S>(pairs[it.pair], it, arena, smem);
+}
+
+template <int T, bool MODE_S>
+__global__ __launch_bounds__(kThreads) void k_inner_intra(const PairDesc *__restrict__ pairs,
+                                                          const WorkItem *__restrict__ items,
+                                                          float *__restrict__ arena, const int *__restrict__ codes,
+                                                          const float *__restrict__ w)
+{
+    __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
+    const WorkItem it = items[blockIdx.x];
+    intra_body<T, MODE_S>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
+}
+
+__global__ __launch_bounds__(kThreads) void k_s1(const PairDesc *__restrict__ pairs, float *__restrict__ arena,
+                                                 const int *__restrict__ codes, const float *__restrict__ w)
+{
+    s1_body(pairs[blockIdx.x], arena, codes, w);
+}
+
+// ------------------------------------------------------- dependency-tile scheduler
+// Every task of a solve (S1, S2 tiles, then per outer diagonal the R0 chunks of every
+// storage tile and the cross-tile chunks and in-tile closures of every logical tile) is
+// one entry of a list in a topological order.  Workgroups take tickets from an atomic
+// head, so a task only ever waits on tasks holding smaller tickets, which are already
+// running or finished: no deadlock, whatever the grid size.  Each logical F / S2 tile has
+// two counters in `flags`:
+//   done    = 1 once its in-tile closure has stored the final values;
+//   pending = number of max-combining contributions (R0 chunks of its storage tile,
+//             cross-tile chunks) still to land; the closure starts when it reaches 0.
+// Waits poll with relaxed agent-scope loads (thread 0 of the workgroup) and end with one
+// acquire fence; producers release with a fence before the counter update.  A wait that
+// exceeds kSpinLimit polls sets *err and abandons the task (and every later waiter gives
+// up when it sees *err), so the kernel always drains.
+constexpr int kSpinLimit = 1 << 22;
+
+__device__ __forceinline__ int ld_relaxed(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct Waiter {
+    const int *err;
+    int polls = 0;
+    bool ok = true;
+    __device__ bool until(const int *p, int want)
+    {
+        while (ok && ld_relaxed(p) != want) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++polls > kSpinLimit || ((polls & 1023) == 0 && ld_relaxed(err) != 0)) ok = false;
+        }
+        return ok;
+    }
+};
+
+// counters of logical tile (X,Y) (nbl logical tiles per side)
+__device__ __forceinline__ int *f_ctr(int *flags, const PairDesc &P, int i1, int j1, int X, int Y)
+{
+    return flags + P.flag + 2 + 2 * tri_ll(P.nbl) + 2 * (utri_ll(i1, j1, P.M) * tri_ll(P.nbl) + utri_ll(X, Y, P.nbl));
+}
+__device__ __forceinline__ int *s_ctr(int *flags, const PairDesc &P, int X, int Y)
+{
+    return flags + P.flag + 2 + 2 * utri_ll(X, Y, P.nbl);
+}
+
+// wait until every upper logical sub-tile of storage tile (SI,SJ) of triangle (i1,j1) is done
+__device__ __forceinline__ void wait_storage(Waiter &W, int *flags, const PairDesc &P, int i1, int j1, int SI, int SJ)
+{
+    const int rs = P.nbl / P.nb;
+    for (int x = SI * rs; x < SI * rs + rs; ++x)
+        for (int y = SJ * rs; y < SJ * rs + rs; ++y)
+            if (x <= y) W.until(f_ctr(flags, P, i1, j1, x, y), 1);
+}
+
+// Thread 0 only: wait for everything task `it` reads.  Returns false on timeout/abort.
+__device__ bool wait_deps(const PairDesc &P, const WorkItem &it, int *flags, const int *err)
+{
+    Waiter W{err};
+    const int type = it.flags >> WI_TYPE_SHIFT;
+    const int I = it.I, J = it.J;
+    switch (type) {
+    case TASK_S1:
+        break;
+    case TASK_S2EXT:
+        for (int s = it.step_begin; s < it.step_end; ++s) {
+            const int K = I + s;
+            W.until(s_ctr(flags, P, I, K), 1);
+            if (K > I) W.until(s_ctr(flags, P, K, J), 1);
+            W.until(s_ctr(flags, P, K + 1, J), 1);
+        }
+        break;
+    case TASK_S2INTRA:
+        W.until(s_ctr(flags, P, I, J) + 1, 0);
+        if (I < J) {
+            W.until(s_ctr(flags, P, I, I), 1);
+            W.until(s_ctr(flags, P, J, J), 1);
+            W.until(s_ctr(flags, P, I + 1, J), 1);
+            W.until(s_ctr(flags, P, I, J - 1), 1);
+            if (J - I >= 2) W.until(s_ctr(flags, P, I + 1, J - 1), 1);
+        }
+        break;
+    case TASK_OUTER: { // storage coordinates
+        const int i1 = it.i1, j1 = it.j1, nK = J - I + 1;
+        W.until(flags + P.flag, 1);                      // S1
+        W.until(flags + P.flag + 1, (int)tri_ll(P.nbl)); // every S2 tile
+        for (int s = it.step_begin; s < it.step_end; ++s) {
+            const int k1 = i1 + s / nK, K = I + s % nK;
+            wait_storage(W, flags, P, i1, k1, I, K);
+            wait_storage(W, flags, P, k1 + 1, j1, K, J);
+            if (K + 1 <= J) wait_storage(W, flags, P, k1 + 1, j1, K + 1, J);
+        }
+        if (it.flags & WI_EPILOGUE) {
+            if (j1 - i1 >= 4) wait_storage(W, flags, P, i1 + 1, j1 - 1, I, J);
+            for (int k1 = i1; k1 < j1; ++k1) {
+                wait_storage(W, flags, P, k1 + 1, j1, I, J);
+                wait_storage(W, flags, P, i1, k1, I, J);
+            }
+        }
+        break;
+    }
+    case TASK_EXT: { // logical coordinates
+        const int D = J - I;
+        W.until(flags + P.flag + 1, (int)tri_ll(P.nbl));
+        for (int s = it.step_begin; s < it.step_end; ++s) {
+            if (s < D) W.until(f_ctr(flags, P, it.i1, it.j1, I, I + s), 1);
+            else W.until(f_ctr(flags, P, it.i1, it.j1, I + 1 + (s - D), J), 1);
+        }
+        break;
+    }
+    case TASK_INTRA:
+        W.until(flags + P.flag + 1, (int)tri_ll(P.nbl));
+        W.until(f_ctr(flags, P, it.i1, it.j1, I, J) + 1, 0);
+        if (I < J) {
+            W.until(f_ctr(flags, P, it.i1, it.j1, I + 1, J), 1);
+            W.until(f_ctr(flags, P, it.i1, it.j1, I, J - 1), 1);
+            if (J - I >= 2) W.until(f_ctr(flags, P, it.i1, it.j1, I + 1, J - 1), 1);
+        }
+        break;
+    }
+    if (!W.ok) atomicOr((int *)err, 1);
+    return W.ok;
+}
+
+// Thread 0 only, after every lane's stores were released.
+__device__ void signal_done(const PairDesc &P, const WorkItem &it, int *flags)
+{
+    const int type = it.flags >> WI_TYPE_SHIFT;
+    switch (type) {
+    case TASK_S1:
+        __hip_atomic_store(flags + P.flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+    case TASK_S2EXT:
+        __hip_atomic_fetch_add(s_ctr(flags, P, it.I, it.J) + 1, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+    case TASK_S2INTRA:
+        __hip_atomic_store(s_ctr(flags, P, it.I, it.J), 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(flags + P.flag + 1, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+    case TASK_OUTER: { // every upper logical sub-tile of the storage tile
+        const int rs = P.nbl / P.nb;
+        for (int x = it.I * rs; x < it.I * rs + rs; ++x)
+            for (int y = it.J * rs; y < it.J * rs + rs; ++y)
+                if (x <= y)
+                    __hip_atomic_fetch_add(f_ctr(flags, P, it.i1, it.j1, x, y) + 1, -1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        break;
+    }
+    case TASK_EXT:
+        __hip_atomic_fetch_add(f_ctr(flags, P, it.i1, it.j1, it.I, it.J) + 1, -1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        break;
+    case TASK_INTRA:
+        __hip_atomic_store(f_ctr(flags, P, it.i1, it.j1, it.I, it.J), 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(kThreads) void k_tasks(const PairDesc *__restrict__ pairs,
+                                                    const WorkItem *__restrict__ tasks, int ntasks, int *head,
+                                                    int *flags, int *err, float *__restrict__ arena,
+                                                    const int *__restrict__ codes, const float *__restrict__ w)
+{
+    constexpr int LT = Inner<T>::LT;
+    constexpr int SM = cmax(Tc<T>::PROD_FLOATS, cmax(Tc<LT>::PROD_FLOATS, Intra<LT>::FLOATS));
+    __shared__ __attribute__((aligned(16))) float smem[SM];
+    __shared__ int s_task, s_ok;
+    if (threadIdx.x == 0) s_task = atomicAdd(head, 1);
+    __syncthreads();
+    const int t = s_task;
+    if (t >= ntasks) return;
+    const WorkItem it = tasks[t];
+    const PairDesc P = pairs[it.pair];
+    if (threadIdx.x == 0) {
+        s_ok = wait_deps(P, it, flags, err);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    switch (it.flags >> WI_TYPE_SHIFT) {
+    case TASK_S1: s1_body(P, arena, codes, w); break;
+    case TASK_S2EXT: ext_body<T, true>(P, it, arena, smem); break;
+    case TASK_S2INTRA: intra_body<T, true>(P, 0, 0, it.I, it.J, arena, codes, w, smem); break;
+    case TASK_OUTER: outer_body<T>(P, it, arena, codes, w, smem); break;
+    case TASK_EXT: ext_body<T, false>(P, it, arena, smem); break;
+    case TASK_INTRA: intra_body<T, false>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem); break;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) signal_done(P, it, flags);
 }
 
 // ------------------------------------------------------------------------- utilities
@@ -436,11 +656,11 @@ __global__ void k_scores(const PairDesc *__restrict__ pairs, int npairs, const f
 // C[i][j] max= max_{i<=k<=j} A[i][k] + B[k][j] on a batch of tiled upper triangles;
 // block = (output tile index within the triangle, matrix).  tiles holds the (I,J) list.
 template <int T>
-__global__ __launch_bounds__(Tc<T>::NT) void k_r0tri(int nb, const int2 *__restrict__ tiles,
-                                                     const float *__restrict__ A,
-                                                     const float *__restrict__ B, float *__restrict__ C)
+__global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restrict__ tiles,
+                                                    const float *__restrict__ A, const float *__restrict__ B,
+                                                    float *__restrict__ C)
 {
-    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD;
+    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD, R = Tc<T>::R;
     __shared__ __attribute__((aligned(16))) float As[T * LD];
     __shared__ __attribute__((aligned(16))) float Bs[T * LD];
     const int2 IJ = tiles[blockIdx.x];
@@ -449,24 +669,28 @@ __global__ __launch_bounds__(Tc<T>::NT) void k_r0tri(int nb, const int2 *__restr
     const float *Am = A + m * tstride, *Bm = B + m * tstride;
     float *Cm = C + m * tstride;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    float acc[4][4];
+    float acc[R][R];
     const float *Ct = tile_at(Cm, IJ.x, IJ.y, nb, T);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float4 c = *(const float4 *)(Ct + (ty * 4 + i) * T + tx * 4);
-        acc[i][0] = c.x;
-        acc[i][1] = c.y;
-        acc[i][2] = c.z;
-        acc[i][3] = c.w;
-    }
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[i][j] = Ct[(ty * R + i) * T + tx * R + j];
+    FetchA<T> fa;
+    FetchB<T> fb;
+    fetch_A<T, T>(fa, Am, nb, IJ.x, IJ.x, 0);
+    fetch_B<T, T>(fb, Bm, nb, IJ.x, IJ.y, 0);
     for (int K = IJ.x; K <= IJ.y; ++K) {
         __syncthreads();
-        load_A<T>(As, Am, nb, IJ.x, K, 0, false);
-        load_B<T>(Bs, Bm, nb, K, IJ.y, 0);
+        commit_A<T>(As, fa, 0, false);
+        commit_B<T>(Bs, fb);
         __syncthreads();
+        if (K + 1 <= IJ.y) {
+            fetch_A<T, T>(fa, Am, nb, IJ.x, K + 1, 0);
+            fetch_B<T, T>(fb, Bm, nb, K + 1, IJ.y, 0);
+        }
         product<T>(acc, As, Bs, ty, tx);
     }
-    store_acc<T>(tile_at(Cm, IJ.x, IJ.y, nb, T), acc, ty, tx, false);
+    store_acc<T, T>(tile_at(Cm, IJ.x, IJ.y, nb, T), acc, ty, tx, false);
 }
 
 // Dense row-major n x n batch <-> tiled triangles (lower cells -> -inf on both sides).
@@ -495,60 +719,59 @@ __global__ void k_tiled_to_dense(int n, int nb, int T, const float *__restrict__
 }
 
 // ---------------------------------------------------------------- launch wrappers
-template <int T>
-static hipError_t outer_T(hipStream_t st, const PairDesc *pairs, const WorkItem *items, int n, float *arena,
-                          const int *codes, const float *w)
-{
-    hipLaunchKernelGGL(k_outer<T>, dim3(n), dim3(Tc<T>::NT), 0, st, pairs, items, arena, codes, w);
-    return hipGetLastError();
-}
-hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n,
-                        float *arena, const int *codes, const float *w)
+#define BPX_T_DISPATCH(T, CALL)                                                                         \
+    switch (T) {                                                                                        \
+    case 32: { constexpr int TT = 32; CALL; } break;                                                    \
+    case 64: { constexpr int TT = 64; CALL; } break;                                                    \
+    default: return hipErrorInvalidValue;                                                               \
+    }
+
+hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n, float *arena,
+                        const int *codes, const float *w)
 {
     if (n <= 0) return hipSuccess;
-    switch (T) {
-    case 32: return outer_T<32>(st, pairs, items, n, arena, codes, w);
-    case 64: return outer_T<64>(st, pairs, items, n, arena, codes, w);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-template <int T, bool S>
-static hipError_t ext_T(hipStream_t st, const PairDesc *pairs, const WorkItem *items, int n, float *arena)
-{
-    hipLaunchKernelGGL((k_inner_ext<T, S>), dim3(n), dim3(Tc<T>::NT), 0, st, pairs, items, arena);
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_outer<TT>, dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
     return hipGetLastError();
 }
+
 hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                             const WorkItem *items, int n, float *arena)
 {
     if (n <= 0) return hipSuccess;
-    if (T == 32) return single_strand ? ext_T<32, true>(st, pairs, items, n, arena) : ext_T<32, false>(st, pairs, items, n, arena);
-    if (T == 64) return single_strand ? ext_T<64, true>(st, pairs, items, n, arena) : ext_T<64, false>(st, pairs, items, n, arena);
-    return hipErrorInvalidValue;
-}
-
-template <int T, bool S>
-static hipError_t intra_T(hipStream_t st, const PairDesc *pairs, const WorkItem *items, int n, float *arena,
-                          const int *codes, const float *w)
-{
-    hipLaunchKernelGGL((k_inner_intra<T, S>), dim3(n), dim3(256), 0, st, pairs, items, arena, codes, w);
+    if (single_strand) {
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_ext<TT, true>), dim3(n), dim3(kThreads), 0, st, pairs, items, arena));
+    } else {
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_ext<TT, false>), dim3(n), dim3(kThreads), 0, st, pairs, items, arena));
+    }
     return hipGetLastError();
 }
+
 hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                               const WorkItem *items, int n, float *arena, const int *codes, const float *w)
 {
     if (n <= 0) return hipSuccess;
-    if (T == 32) return single_strand ? intra_T<32, true>(st, pairs, items, n, arena, codes, w) : intra_T<32, false>(st, pairs, items, n, arena, codes, w);
-    if (T == 64) return single_strand ? intra_T<64, true>(st, pairs, items, n, arena, codes, w) : intra_T<64, false>(st, pairs, items, n, arena, codes, w);
-    return hipErrorInvalidValue;
+    if (single_strand) {
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, true>), dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
+    } else {
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, false>), dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_s1(hipStream_t st, const PairDesc *pairs, int npairs, float *arena, const int *codes,
                      const float *w)
 {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_s1, dim3(npairs), dim3(256), 0, st, pairs, arena, codes, w);
+    hipLaunchKernelGGL(k_s1, dim3(npairs), dim3(kThreads), 0, st, pairs, arena, codes, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_tasks(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *tasks, int ntasks, int *head,
+                        int *flags, int *err, float *arena, const int *codes, const float *w)
+{
+    if (ntasks <= 0) return hipSuccess;
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_tasks<TT>, dim3(ntasks), dim3(kThreads), 0, st, pairs, tasks, ntasks,
+                                         head, flags, err, arena, codes, w));
     return hipGetLastError();
 }
 
@@ -582,13 +805,7 @@ hipError_t launch_r0tri(hipStream_t st, int T, int nb, const int2 *tiles, int nt
                         const float *B, float *C)
 {
     if (ntiles <= 0 || batch <= 0) return hipSuccess;
-    if (T == 64) {
-        hipLaunchKernelGGL(k_r0tri<64>, dim3(ntiles, batch), dim3(Tc<64>::NT), 0, st, nb, tiles, A, B, C);
-    } else if (T == 32) {
-        hipLaunchKernelGGL(k_r0tri<32>, dim3(ntiles, batch), dim3(Tc<32>::NT), 0, st, nb, tiles, A, B, C);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_r0tri<TT>, dim3(ntiles, batch), dim3(kThreads), 0, st, nb, tiles, A, B, C));
     return hipGetLastError();
 }
 hipError_t launch_dense_to_tiled(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst)

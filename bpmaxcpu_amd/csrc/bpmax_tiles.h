@@ -1,0 +1,232 @@
+// bpmax_tiles.h -- device-side tile machinery shared by the BPMax kernels.
+//
+// Storage: every inner triangle (an outer triangle's N x N upper triangle, or S2) is kept
+// as square T x T storage tiles, row-major, in row-major upper-tile order (see
+// bpmax_internal.h).  Phases address it through "logical" LT x LT tiles (LT divides T):
+// logical tile (X,Y) is the LT x LT block at storage tile (X/RS, Y/RS), offset
+// ((X%RS)*LT, (Y%RS)*LT), row stride T, RS = T/LT.  R0 runs on LT = T (64 x 64, 4 x 4
+// register tiles); the inner closure (cross-tile R1/R2, in-tile wavefront, S2) runs on
+// LT = 32, which halves the wavefront's dependent steps and quarters its per-step work.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bpmax_internal.h"
+
+namespace bpx {
+
+#define NEG_INF (-__builtin_inff())
+
+constexpr int kThreads = 256; // every phase body runs on 256 lanes (4 waves)
+
+template <int LT>
+struct Tc {
+    static constexpr int R = LT / 16;   // register tile edge: each lane owns R x R cells
+    static constexpr int TX = 16;       // lanes per tile row
+    static constexpr int LD = LT + 4;   // LDS row stride of product operands (16-byte rows)
+    static constexpr int PROD_FLOATS = 2 * LT * LD;
+};
+
+// Storage tile (I,J) (runtime T), for the layout conversions.
+__device__ __forceinline__ const float *tile_at(const float *tri, int I, int J, int nb, int T)
+{
+    return tri + utri_ll(I, J, nb) * (long long)(T * T);
+}
+__device__ __forceinline__ float *tile_at(float *tri, int I, int J, int nb, int T)
+{
+    return tri + utri_ll(I, J, nb) * (long long)(T * T);
+}
+
+__device__ __forceinline__ float *tri_at(float *arena, const PairDesc &P, int i1, int j1)
+{
+    return arena + P.F + utri_ll(i1, j1, P.M) * P.tstride;
+}
+
+// Logical LT x LT tile (X,Y) of a triangle with nb storage tiles per side (X <= Y).
+template <int T, int LT>
+__device__ __forceinline__ float *ltile(const float *tri, int X, int Y, int nb)
+{
+    constexpr int RS = T / LT;
+    return (float *)tri + utri_ll(X / RS, Y / RS, nb) * (long long)(T * T) + (X % RS) * LT * T + (Y % RS) * LT;
+}
+
+// Tile staging in two phases so that global-memory latency overlaps compute: fetch_*
+// issues every global load of a tile into registers (unrolled, branch-free: an absent
+// source tile is read from a valid address and replaced by -inf), commit_* writes the
+// registers to LDS.  PER = float4 loads per lane per tile.
+template <int LT>
+struct Stage {
+    static constexpr int Q = LT / 4, PER = LT * Q / kThreads;
+};
+
+// A operand: As[k][r] = X[I*LT + r][K*LT + k + cshift] (cshift 0 or -1), stored k-major
+// for the product's vector row reads.  -inf where the source tile lies below the diagonal;
+// with lastcol_only every column but k = LT-1 is -inf.  Lane mapping: within a wave, 16
+// consecutive rows x 4 consecutive float4 column chunks, which makes the transposed LDS
+// stores conflict-free and keeps 64-byte row runs in each global load.
+template <int LT>
+struct FetchA {
+    float4 v[Stage<LT>::PER];
+    float prev;          // X[I*LT + r][K*LT - 1] for lane r < LT (cshift = -1)
+    bool have, have_prev;
+};
+
+template <int LT>
+__device__ __forceinline__ void a_coords(int p, int &r, int &c4)
+{
+    const int idx = threadIdx.x + p * kThreads;
+    const int B = idx / 64, l = idx % 64;
+    r = 16 * (B % (LT / 16)) + l % 16;
+    c4 = 4 * (4 * (B / (LT / 16)) + l / 16);
+}
+
+template <int T, int LT>
+__device__ __forceinline__ void fetch_A(FetchA<LT> &f, const float *X, int nb, int I, int K, int cshift)
+{
+    const int nbl = nb * (T / LT);
+    f.have = K >= I && K < nbl;
+    const float *t = ltile<T, LT>(X, I, f.have ? K : I, nb);
+#pragma unroll
+    for (int p = 0; p < Stage<LT>::PER; ++p) {
+        int r, c4;
+        a_coords<LT>(p, r, c4);
+        f.v[p] = *(const float4 *)(t + r * T + c4);
+    }
+    f.have_prev = cshift < 0 && K - 1 >= I;
+    f.prev = NEG_INF;
+    if (cshift < 0) {
+        const float *tp = ltile<T, LT>(X, I, f.have_prev ? K - 1 : I, nb);
+        f.prev = tp[(threadIdx.x % LT) * T + LT - 1];
+    }
+}
+
+template <int LT>
+__device__ __forceinline__ void commit_A(float *As, const FetchA<LT> &f, int cshift, bool lastcol_only)
+{
+    constexpr int LD = Tc<LT>::LD;
+#pragma unroll
+    for (int p = 0; p < Stage<LT>::PER; ++p) {
+        int r, c4;
+        a_coords<LT>(p, r, c4);
+        const float e[4] = {f.v[p].x, f.v[p].y, f.v[p].z, f.v[p].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = c4 + q - cshift;
+            float val = f.have ? e[q] : NEG_INF;
+            if (lastcol_only && k != LT - 1) val = NEG_INF;
+            if (k < LT) As[k * LD + r] = val;
+        }
+    }
+    if (cshift < 0 && threadIdx.x < LT) As[threadIdx.x] = (f.have_prev && !lastcol_only) ? f.prev : NEG_INF;
+}
+
+// B operand: Bs[k][c] = X[K*LT + k + rshift][J*LT + c] (rshift 0 or +1); rows of tiles
+// below the diagonal (or past the last tile) are -inf.
+template <int LT>
+struct FetchB {
+    float4 v[Stage<LT>::PER];
+    unsigned have; // bit p: row of chunk p exists
+};
+
+template <int T, int LT>
+__device__ __forceinline__ void fetch_B(FetchB<LT> &f, const float *X, int nb, int K, int J, int rshift)
+{
+    constexpr int Q = Stage<LT>::Q;
+    const int nbl = nb * (T / LT);
+    f.have = 0;
+#pragma unroll
+    for (int p = 0; p < Stage<LT>::PER; ++p) {
+        const int idx = threadIdx.x + p * kThreads;
+        const int k = idx / Q, c4 = (idx % Q) * 4;
+        const int x = K * LT + k + rshift;
+        const int Xt = x / LT;
+        const bool ok = Xt <= J && Xt < nbl;
+        f.have |= ok ? (1u << p) : 0u;
+        const float *t = ltile<T, LT>(X, ok ? Xt : J, J, nb);
+        f.v[p] = *(const float4 *)(t + (ok ? x % LT : 0) * T + c4);
+    }
+}
+
+template <int LT>
+__device__ __forceinline__ void commit_B(float *Bs, const FetchB<LT> &f)
+{
+    constexpr int LD = Tc<LT>::LD, Q = Stage<LT>::Q;
+    const float4 ninf = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
+#pragma unroll
+    for (int p = 0; p < Stage<LT>::PER; ++p) {
+        const int idx = threadIdx.x + p * kThreads;
+        const int k = idx / Q, c4 = (idx % Q) * 4;
+        *(float4 *)(Bs + k * LD + c4) = ((f.have >> p) & 1u) ? f.v[p] : ninf;
+    }
+}
+
+// acc[i][j] = max(acc[i][j], max_k As[k][ty*R+i] + Bs[k][tx*R+j]) over the whole tile depth:
+// two k per round, v_add_f32 x2 + v_max3_f32 per pair of cell updates.
+template <int LT>
+__device__ __forceinline__ void product(float (&acc)[Tc<LT>::R][Tc<LT>::R], const float *As, const float *Bs,
+                                        int ty, int tx)
+{
+    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R;
+#pragma unroll 4
+    for (int k = 0; k < LT; k += 2) {
+        float av0[R], av1[R], bv0[R], bv1[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            av0[i] = As[k * LD + ty * R + i];
+            av1[i] = As[(k + 1) * LD + ty * R + i];
+            bv0[i] = Bs[k * LD + tx * R + i];
+            bv1[i] = Bs[(k + 1) * LD + tx * R + i];
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                acc[i][j] = fmaxf(acc[i][j], fmaxf(av0[i] + bv0[j], av1[i] + bv1[j]));
+    }
+}
+
+// Max-combine the lane's R x R cells into tile `dst` (row stride T) -- every defined cell's
+// final value is >= 0, where float order equals the signed-integer order of the bit
+// patterns -- or store them.
+template <int T, int LT>
+__device__ __forceinline__ void store_acc(float *dst, const float (&acc)[Tc<LT>::R][Tc<LT>::R], int ty, int tx,
+                                          bool atomic)
+{
+    constexpr int R = Tc<LT>::R;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        float *row = dst + (ty * R + i) * T + tx * R;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (atomic) atomicMax((int *)(row + j), __float_as_int(acc[i][j]));
+            else row[j] = acc[i][j];
+        }
+    }
+}
+
+// Max over aligned groups of G consecutive lanes: quad permutes on the DPP path (no LDS)
+// for the first two levels, a lane shuffle beyond.
+template <int G>
+__device__ __forceinline__ float lane_group_max(float v)
+{
+    if (G >= 2) v = fmaxf(v, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false)));
+    if (G >= 4) v = fmaxf(v, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false)));
+    // row_half_mirror: lane i <-> 7-i inside each half-row, pairs the two quads of 8 lanes
+    if (G >= 8) v = fmaxf(v, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false)));
+#pragma unroll
+    for (int off = 8; off < G; off <<= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// LDS layout of the in-tile closure on an LT x LT tile: G lanes per cell, U accumulators
+// per lane, PAD = U*G extra -inf rows after Xs and Sb (see intra_body), then the tile's
+// P2 weights wt[a][b] = intra(b[r], b[c]) and the boundary rows/columns.
+template <int LT>
+struct Intra {
+    static constexpr int G = kThreads / LT, U = 32 / G, PAD = U * G, L = LT + 4;
+    static constexpr int XS = 0, SA = (LT + PAD) * L, SB = SA + LT * L, WT = SB + (LT + PAD) * L;
+    static constexpr int BELOW = WT + LT * LT, WL = BELOW + 2 * LT + 4, CODES = WL + 28;
+    static constexpr int FLOATS = CODES + 2 * LT;
+};
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+} // namespace bpx

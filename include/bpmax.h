@@ -150,6 +150,7 @@ typedef enum {
     BPMAX_K_INNER_INTRA,/* intra-tile wavefront (R1/R2/P2 inside a tile; S2's)       */
     BPMAX_K_CONVERT,
     BPMAX_K_R0TRI,      /* isolated R0 kernel                                          */
+    BPMAX_K_TASKS,      /* dependency-tile scheduler: every phase of a solve in one launch */
     BPMAX_K_NCLASSES
 } bpmax_kernel_class;
 /* Returns launches, total milliseconds and total algorithmic cell updates for a class
@@ -157,6 +158,25 @@ typedef enum {
 int bpmax_ctx_kernel_stats(bpmax_ctx *ctx, int kclass, long long *launches, double *ms,
                            double *updates);
 int bpmax_ctx_reset_stats(bpmax_ctx *ctx);
+/* Schedules:
+ *   BPMAX_SCHED_PIPE (default): one launch per phase and block diagonal on two streams --
+ *     R0 work of outer diagonal d1 on one, the inner closure of d1 on the other, joined by
+ *     events at the tile diagonals each launch reads -- so the closure of d1 overlaps the R0
+ *     work of d1 and d1+1.  Without timing the whole launch DAG is captured once per
+ *     problem shape into a HIP graph and replayed (bpmax_ctx_set_graphs).
+ *   BPMAX_SCHED_LAUNCH: the same launches on one stream, in reference fork/join order
+ *     (per-phase timing without overlap).
+ *   BPMAX_SCHED_TASKS: a whole solve as one launch of the dependency-tile scheduler; every
+ *     tile task waits on per-tile ready counters of the tiles it reads.
+ * chunk = K-steps per task of the task scheduler (0 keeps the current value).  The
+ * BPMAX_SCHEDULE=pipe|launch|tasks environment variable sets the default. */
+#define BPMAX_SCHED_TASKS 0
+#define BPMAX_SCHED_LAUNCH 1
+#define BPMAX_SCHED_PIPE 2
+int bpmax_ctx_set_schedule(bpmax_ctx *ctx, int schedule, int chunk);
+/* Enable or disable (default) HIP-graph capture and replay for BPMAX_SCHED_PIPE; without
+ * timing only.  BPMAX_GRAPHS=1 in the environment sets the default. */
+int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);
 /* The hipStream_t the context launches on (as void*). */
 void *bpmax_ctx_stream(bpmax_ctx *ctx);
 int bpmax_ctx_synchronize(bpmax_ctx *ctx);

@@ -29,6 +29,17 @@ def eng():
     e.close()
 
 
+@pytest.fixture(scope="module", params=["pipe", "pipe-eager", "launch", "tasks"])
+def eng2(request):
+    """An engine under each schedule: two-stream pipeline (graph-replayed and eager),
+    per-phase launches on one stream, and the dependency-tile scheduler."""
+    e = B.Engine(0)
+    e.set_schedule(request.param.split("-")[0])
+    e.set_graphs(request.param != "pipe-eager")
+    yield e
+    e.close()
+
+
 def _bits_equal(a, b):
     a = np.asarray(a, np.float32)
     b = np.asarray(b, np.float32)
@@ -36,19 +47,19 @@ def _bits_equal(a, b):
 
 
 @pytest.mark.parametrize("path", FULL, ids=[G.fid(p) for p in FULL])
-def test_full_fixture(eng, path):
+def test_full_fixture(eng2, path):
     fx = G.load_full(path)
     _say(f"full fixture {G.fid(path)}")
-    F, score = eng.solve(fx["seq1"], fx["seq2"], fx["intra"], fx["inter"])
+    F, score = eng2.solve(fx["seq1"], fx["seq2"], fx["intra"], fx["inter"])
     assert score == fx["score"]
     assert _bits_equal(F, fx["F"])
 
 
 @pytest.mark.parametrize("path", HASHED, ids=[G.fid(p) for p in HASHED])
-def test_hash_fixture(eng, path):
+def test_hash_fixture(eng2, path):
     fx = G.load_hash(path)
     _say(f"hash fixture {G.fid(path)} M={fx['M']} N={fx['N']}")
-    F, score = eng.solve(fx["seq1"], fx["seq2"], fx["intra"], fx["inter"])
+    F, score = eng2.solve(fx["seq1"], fx["seq2"], fx["intra"], fx["inter"])
     assert score == fx["score"]
     for i1, j1, i2, j2, v in fx["samples"]:
         assert F[O.packed_index(i1, j1, i2, j2, fx["M"], fx["N"])] == v
@@ -62,13 +73,13 @@ EDGE = [(1, 1), (1, 2), (2, 1), (1, 5), (5, 1), (3, 4), (4, 5), (6, 8), (2, 31),
 
 
 @pytest.mark.parametrize("M,N", EDGE)
-def test_edge_shapes_vs_oracle(eng, M, N):
+def test_edge_shapes_vs_oracle(eng2, M, N):
     rng = np.random.default_rng(1000 * M + N)
     s1 = rng.integers(0, 5, M).astype(np.int32)   # includes code 4 (N, never pairs)
     s2 = rng.integers(0, 5, N).astype(np.int32)
     intra, inter = O.table_asym()
     _say(f"edge M={M} N={N}")
-    F, score = eng.solve(s1, s2, intra, inter)
+    F, score = eng2.solve(s1, s2, intra, inter)
     Fo, so = O.bpmax(s1, s2, intra, inter)
     assert score == so
     assert _bits_equal(F, Fo)
@@ -110,7 +121,8 @@ def test_dense_layout_matches_packed(eng):
                 assert np.all(Fd[i1, j1, i2, :i2] == 0)
 
 
-def test_batch_matches_single_and_oracle(eng):
+def test_batch_matches_single_and_oracle(eng2):
+    eng = eng2
     rng = np.random.default_rng(77)
     pairs = [(int(rng.integers(1, 12)), int(rng.integers(1, 150))) for _ in range(12)]
     s1s = [rng.integers(0, 4, m).astype(np.int32) for m, _ in pairs]
@@ -232,3 +244,25 @@ def test_compat_bpmax_symbol_writes_6d_tree(eng):
                     v = lin[off + (ci % N_tile) * N_tile + cj % N_tile]
                     assert v == F[k], (i1, j1, i2, j2)
                     k += 1
+
+
+def test_graph_replays_match_oracle(eng):
+    """The pipelined schedule's graph is captured once per shape and replayed: every replay,
+    with eager work and synchronisations between replays, and a shape change in the middle
+    (graph rebuilt, then rebuilt back), gives the oracle's table bit for bit."""
+    rng = np.random.default_rng(11)
+    intra, inter = O.table_asym()
+    shapes = [(5, 70), (5, 70), (5, 70), (3, 33), (5, 70), (5, 70)]
+    for k, (M, N) in enumerate(shapes):
+        s1 = rng.integers(0, 4, M).astype(np.int32)
+        s2 = rng.integers(0, 4, N).astype(np.int32)
+        _say(f"graph replay {k} M={M} N={N}")
+        F, score = eng.solve(s1, s2, intra, inter)
+        Fo, so = O.bpmax(s1, s2, intra, inter)
+        assert score == so
+        assert _bits_equal(F, Fo)
+        # eager work on the same stream between replays
+        eng.set_timing(True)
+        eng.solve(s1[:2], s2[:9], intra, inter)
+        eng.set_timing(False)
+        eng.synchronize()
