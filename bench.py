@@ -1,21 +1,25 @@
-"""BPMax benchmark on one MI355X: GCUPS of one full F-table computation at M=64, N=512.
+"""BPMax benchmark on one MI355X.
 
-A step is one complete BPMax solve of one 64x512 pair (S1, S2, every outer diagonal's
-R0/R3/R4/P1 phase and every inner R1/R2/P2 phase) with sequences already on the host and
-the F table left resident in HBM (score-only output).  GCUPS = cells(M,N) / step time,
+Headline: GCUPS of one full F-table computation at M=64, N=512.  A step is one complete
+BPMax solve of one 64x512 pair (S1, S2, every outer diagonal's R0/R3/R4/P1 phase and every
+inner R1/R2/P2 phase) with the sequences on the host and the F table left resident in HBM
+(score-only output).  GCUPS = cells(M,N) / step time with
 cells = C(M+1,3)C(N+1,3) + 2C(M+1,2)C(N+1,3) + 2C(M+1,3)C(N+1,2).
 
-Besides the JSON line's headline value the line carries
-  roofline     -- the dominant kernel (k_outer: R0 + init/P1/R3/R4) from HIP events on the
-                  engine's stream: algorithmic flops (2 per cell update) / kernel time,
-                  against the MI355X FP32 vector peak;
+The JSON line also carries
+  roofline     -- the dominant kernel, k_outer<64> (R0 + init/P1/R3/R4), timed with HIP
+                  events on the stream it is launched on during the timed steps:
+                  algorithmic flops (2 per cell update: one add, one max) per launch over
+                  its average launch duration, against the MI355X FP32 vector peak;
   cpu_baseline -- the reference's own bpmax() (oracle/_ref, compiled from the reference
-                  sources) on this host's cores for the same 64x512 pair.
+                  sources) on this host's cores for the same 64x512 pair;
+  batched      -- 256 mixed-size pairs (M~U[16,48], N~U[100,400], GC~U[0.3,0.7]) solved
+                  in one batched schedule vs the same pairs solved one per call;
+  r0_isolated  -- the R0 max-plus kernel alone on 4096 tiled 512x512 upper triangles.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -24,8 +28,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-FP32_VALU_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector peak (counts FMA as 2 flops)
-MAXPLUS_CEILING_TFLOPS = 78.6     # one add + one max per lane-slot per clock
+FP32_VALU_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector peak (packed FMA counted as 4 flops/lane)
 
 
 def log(msg):
@@ -37,17 +40,33 @@ def workload(M, N, seed=7):
     return rng.integers(0, 4, M).astype(np.int32), rng.integers(0, 4, N).astype(np.int32)
 
 
+def gc_sequence(rng, n, gc):
+    """i.i.d. bases with P(G)=P(C)=gc/2, P(A)=P(U)=(1-gc)/2 (codes A0 C1 G2 U3)."""
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return rng.choice(4, size=n, p=p).astype(np.int32)
+
+
+def mixed_pairs(npairs=256, seed=11):
+    rng = np.random.default_rng(seed)
+    s1, s2 = [], []
+    for _ in range(npairs):
+        M = int(rng.integers(16, 49))
+        N = int(rng.integers(100, 401))
+        s1.append(gc_sequence(rng, M, rng.uniform(0.3, 0.7)))
+        s2.append(gc_sequence(rng, N, rng.uniform(0.3, 0.7)))
+    return s1, s2
+
+
 def cpu_baseline(M, N, s1, s2, threads):
     """Reference bpmax() on the host (kind 'reference'), or the oracle restatement on a
     bounded sample when the reference binary is absent (kind 'port')."""
     from oracle import oracle as O
-    cells = None
     import bpmaxcpu_amd as B
     if O.have_reference():
         log(f"cpu baseline: reference bpmax() {M}x{N} on {threads} threads")
         _, score, secs, _ = O.reference_run(s1, s2, table=False, threads=threads)
         cells = B.cell_updates(M, N)
-        return {"value": cells / secs / 1e9, "unit": "GCUPS", "cores": threads, "kind": "reference",
+        return {"value": round(cells / secs / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "reference",
                 "sample": f"one {M}x{N} pair (the full workload), reference [MPT+RT]:v3 bpmax() built with "
                           f"g++ -O3 -fopenmp -mavx2 -mfma + FTZ, N_tile 48, 3x24 register tile, "
                           f"OMP_NUM_THREADS={threads}; {secs:.2f} s, score {score}"}
@@ -57,8 +76,127 @@ def cpu_baseline(M, N, s1, s2, threads):
     t0 = time.time()
     O.bpmax(s1[:Ms], s2)
     secs = time.time() - t0
-    return {"value": B.cell_updates(Ms, N) / secs / 1e9, "unit": "GCUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle restatement on the first {Ms} bases of seq1 x all {N} of seq2; {secs:.2f} s"}
+    return {"value": round(B.cell_updates(Ms, N) / secs / 1e9, 3), "unit": "GCUPS", "cores": threads,
+            "kind": "port", "sample": f"oracle restatement on the first {Ms} bases of seq1 x all {N} of seq2; "
+                                      f"{secs:.2f} s"}
+
+
+def pair_leg(B, eng, M, N, steps, warmup):
+    s1, s2 = workload(M, N)
+    cells = B.cell_updates(M, N)
+    log(f"warmup {warmup} x {M}x{N}")
+    score = None
+    for _ in range(warmup):
+        _, score = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    eng.synchronize()
+    # timed steps, HIP events around every R0 launch on the stream it runs on
+    eng.set_timing(True, classes=["outer"])
+    eng.reset_stats()
+    log(f"timed {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, score = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    st = eng.kernel_stats()
+    eng.set_timing(False)
+    ms_per_step = wall * 1e3 / steps
+    gcups = cells / (ms_per_step / 1e3) / 1e9
+    # the same steps without events, to show what the instrumentation costs
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    eng.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / steps
+    log(f"{ms_per_step:.2f} ms/step with R0 events ({ms_plain:.2f} without), {gcups:.1f} GCUPS, score {score}")
+    # every class timed (costs ~15% on this many launches): per-class durations while overlapped
+    eng.set_timing(True)
+    eng.reset_stats()
+    eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    phases = {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in eng.kernel_stats().items()
+              if v["launches"]}
+    eng.set_timing(False)
+
+    o = st["outer"]
+    flops_per_launch = 2.0 * o["updates"] / max(1, o["launches"])
+    avg_s = o["ms"] / max(1, o["launches"]) / 1e3
+    achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_outer<64>: R0 double max-plus + init/P1/R3/R4 (2 flops per cell update)",
+                "launches": o["launches"], "avg_launch_ms": round(avg_s * 1e3, 4),
+                "flops_per_launch": flops_per_launch}
+
+    # one-stream schedule, for the phase breakdown without overlap
+    eng.set_schedule(B.SCHED_LAUNCH)
+    eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    eng.set_timing(True)
+    eng.reset_stats()
+    eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    st2 = eng.kernel_stats()
+    eng.set_timing(False)
+    eng.set_schedule(B.SCHED_PIPE)
+    serial = {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in st2.items() if v["launches"]}
+    o2 = st2["outer"]
+    r0_serial = 2.0 * o2["updates"] / (o2["ms"] / 1e3) / 1e12 if o2["ms"] > 0 else 0.0
+    roofline["k_outer_alone"] = {"achieved": round(r0_serial, 3), "frac": round(r0_serial / FP32_VALU_PEAK_TFLOPS, 4),
+                                 "note": "one-stream schedule, one R0 launch per outer diagonal, no overlap"}
+    log(f"k_outer {achieved:.2f} TFLOP/s in the pipelined run, {r0_serial:.2f} alone; phases {serial}")
+    return {"score": score, "ms_per_step": ms_per_step, "ms_plain": ms_plain, "gcups": gcups,
+            "roofline": roofline, "phases_pipelined": phases, "phases_one_stream": serial}
+
+
+def batched_leg(B, eng, reps=2):
+    s1, s2 = mixed_pairs()
+    n = len(s1)
+    cells = sum(B.cell_updates(len(a), len(b)) for a, b in zip(s1, s2))
+    log(f"batched: {n} mixed pairs, {cells / 1e12:.2f} T cells")
+    eng.solve_batch(s1, s2)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        sb = eng.solve_batch(s1, s2)
+        ts.append(time.perf_counter() - t0)
+    tb = min(ts)
+    log(f"batched {tb * 1e3:.1f} ms ({n / tb:.1f} pairs/s); now one solve per pair")
+    t0 = time.perf_counter()
+    single = np.array([eng.solve(a, b, layout=B.LAYOUT_NONE)[1] for a, b in zip(s1, s2)], np.float32)
+    ts_ = time.perf_counter() - t0
+    same = bool(np.array_equal(single, sb))
+    log(f"per-pair {ts_ * 1e3:.1f} ms ({n / ts_:.1f} pairs/s); scores identical: {same}")
+    return {"workload": f"{n} pairs, M~U[16,48], N~U[100,400], GC~U[0.3,0.7], seed 11",
+            "pairs_per_s": round(n / tb, 2), "gcups": round(cells / tb / 1e9, 2), "ms": round(tb * 1e3, 2),
+            "per_pair_calls": {"pairs_per_s": round(n / ts_, 2), "ms": round(ts_ * 1e3, 2)},
+            "speedup_vs_per_pair": round(ts_ / tb, 3), "scores_identical": same}
+
+
+def r0_leg(B, eng, n=512, batch=4096, reps=3):
+    tri = eng.tri_tiled_floats(n)
+    log(f"r0 isolated: {batch} x {n}x{n} tiled triangles ({3 * tri * batch * 4 / 2**30:.1f} GiB)")
+    A = eng.tri_alloc(n, batch)
+    Bm = eng.tri_alloc(n, batch)
+    C = eng.tri_alloc(n, batch)
+    try:
+        eng.tri_fill_random(A, n, batch, 1, 0, 200)
+        eng.tri_fill_random(Bm, n, batch, 2, 0, 200)
+        eng.tri_fill_random(C, n, batch, 3, 0, 200)
+        eng.r0_tri_batch(n, batch, A, Bm, C)
+        eng.set_timing(True)
+        eng.reset_stats()
+        for _ in range(reps):
+            eng.r0_tri_batch(n, batch, A, Bm, C)
+        st = eng.kernel_stats()["r0tri"]
+        eng.set_timing(False)
+        avg_ms = st["ms"] / st["launches"]
+        upd = st["updates"] / st["launches"]
+        tf = 2.0 * upd / (avg_ms / 1e3) / 1e12
+        log(f"r0 isolated {avg_ms:.2f} ms/launch, {tf:.2f} TFLOP/s")
+        return {"workload": f"{batch} upper-triangular {n}x{n} fp32 triples, integers U[0,200], -inf below",
+                "ms_per_launch": round(avg_ms, 3), "achieved": round(tf, 3), "unit": "TFLOP/s",
+                "frac": round(tf / FP32_VALU_PEAK_TFLOPS, 4), "kernel": "k_r0tri<64>"}
+    finally:
+        for p in (A, Bm, C):
+            eng.tri_free(p)
 
 
 def main():
@@ -69,8 +207,22 @@ def main():
     ap.add_argument("--M", type=int, default=64)
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and isolated-R0 legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel over MIB MiB (reads MIB, writes MIB): the "
+                         "reference point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = (args.calibrate_copy << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; pairs are independent, so a multi-GPU run is "
               "pair-level data parallelism with no communication (not measured here).", file=sys.stderr)
@@ -78,67 +230,31 @@ def main():
 
     import bpmaxcpu_amd as B
     M, N = args.M, args.N
-    s1, s2 = workload(M, N)
-    cells = B.cell_updates(M, N)
     eng = B.Engine(0)
-    log(f"warmup {args.warmup} x {M}x{N}")
-    score = None
-    for _ in range(args.warmup):
-        _, score = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
-    eng.synchronize()
-    log(f"timed {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, score = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
-    eng.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    gcups = cells / (ms_per_step / 1e3) / 1e9
-    log(f"{ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS, score {score}")
-
-    # instrumented steps: HIP events on the engine stream around every launch
-    eng.set_timing(True)
-    eng.reset_stats()
-    eng.solve(s1, s2, layout=B.LAYOUT_NONE)
-    st = eng.kernel_stats()
-    tk = st["tasks"]
-    achieved = 2.0 * tk["updates"] / (tk["ms"] / 1e3) / 1e12 if tk["ms"] > 0 else 0.0
-    roofline = {"bound": "valu", "kernel": "k_tasks<64> (dependency-tile scheduler: every phase of the solve)",
-                "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4),
-                "frac_of_maxplus_ceiling": round(achieved / MAXPLUS_CEILING_TFLOPS, 4),
-                "launches": tk["launches"], "avg_launch_ms": round(tk["ms"] / max(1, tk["launches"]), 4),
-                "flops_per_launch": 2.0 * tk["updates"] / max(1, tk["launches"]),
-                "traffic": None}
-    # the same solve with one launch per phase and diagonal, to isolate the R0 phase
-    eng.set_schedule(B.SCHED_LAUNCH)
-    eng.reset_stats()
-    eng.solve(s1, s2, layout=B.LAYOUT_NONE)
-    st2 = eng.kernel_stats()
-    eng.set_schedule(B.SCHED_TASKS)
-    eng.set_timing(False)
-    o = st2["outer"]
-    r0 = 2.0 * o["updates"] / (o["ms"] / 1e3) / 1e12 if o["ms"] > 0 else 0.0
-    roofline["r0_phase"] = {"kernel": "k_outer<64> (R0 + init/P1/R3/R4, one launch per outer diagonal)",
-                            "achieved": round(r0, 3), "frac": round(r0 / FP32_VALU_PEAK_TFLOPS, 4),
-                            "frac_of_maxplus_ceiling": round(r0 / MAXPLUS_CEILING_TFLOPS, 4),
-                            "launches": o["launches"], "ms": round(o["ms"], 3)}
-    phases = {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in st2.items() if v["launches"]}
-    log(f"k_tasks {tk['ms']:.2f} ms ({achieved:.1f} TFLOP/s); per-phase launches: {phases}")
+    res = pair_leg(B, eng, M, N, args.steps, args.warmup)
+    batched = r0 = None
+    if not args.no_extra:
+        batched = batched_leg(B, eng)
+        r0 = r0_leg(B, eng)
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1, 128)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        s1, s2 = workload(M, N)
         cpu = cpu_baseline(M, N, s1, s2, threads)
         log(f"cpu baseline {cpu['value']:.2f} GCUPS on {threads} threads")
 
-    line = {"metric": "BPMax GCUPS at M=64,N=512", "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+    line = {"metric": "BPMax GCUPS at M=64,N=512", "value": round(res["gcups"], 3), "unit": "GCUPS", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["ms_per_step"], 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic: uniform i.i.d. ACGU from seed 7, reference weight table (GC=3, AU=1, GU=1.5)",
             "config": {"workload": f"plain BPMax, one pair M={M} N={N}, score-only output, F resident in HBM",
-                       "M": M, "N": N, "pairs_per_step": 1, "score": score},
-            "roofline": roofline, "cpu_baseline": cpu, "launch_schedule_phases_ms": phases}
+                       "M": M, "N": N, "pairs_per_step": 1, "score": res["score"],
+                       "schedule": "two-stream pipeline, 2 R0 launches per outer diagonal"},
+            "roofline": res["roofline"], "cpu_baseline": cpu,
+            "ms_per_step_without_events": round(res["ms_plain"], 3),
+            "phases_pipelined": res["phases_pipelined"], "phases_one_stream": res["phases_one_stream"],
+            "batched": batched, "r0_isolated": r0}
     print(json.dumps(line), flush=True)
     eng.close()
 

@@ -68,9 +68,11 @@ def load_library(path: str = LIB_PATH):
         "bpmax_r0_tri_batch_device": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
         "bpmax_tri_upload": (c_int, [c_void_p, c_int, c_int, fp, c_void_p]),
         "bpmax_tri_download": (c_int, [c_void_p, c_int, c_int, c_void_p, fp]),
+        "bpmax_tri_fill_random": (c_int, [c_void_p, c_int, c_int, ctypes.c_ulonglong, c_int, c_int, c_void_p]),
         "bpmax_device_alloc": (c_void_p, [c_void_p, c_size_t]),
         "bpmax_device_free": (c_int, [c_void_p, c_void_p]),
         "bpmax_ctx_set_timing": (c_int, [c_void_p, c_int]),
+        "bpmax_ctx_set_timing_mask": (c_int, [c_void_p, ctypes.c_uint]),
         "bpmax_ctx_kernel_stats": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_longlong),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "bpmax_ctx_reset_stats": (c_int, [c_void_p]),
@@ -199,6 +201,9 @@ class Engine:
         return scores
 
     # ------------------------------------------------------- isolated R0 kernel
+    def tri_tiled_floats(self, n: int) -> int:
+        return self.L.bpmax_tri_tiled_floats(n)
+
     def tri_alloc(self, n: int, batch: int) -> int:
         floats = self.L.bpmax_tri_tiled_floats(n) * batch
         p = self.L.bpmax_device_alloc(self.h, floats * 4)
@@ -219,6 +224,10 @@ class Engine:
         self._check(self.L.bpmax_tri_download(self.h, n, batch, dev, _fp(out)))
         return out
 
+    def tri_fill_random(self, dev: int, n: int, batch: int, seed: int, lo: int, hi: int):
+        """Synthetic integer operands in [lo, hi] on/above the diagonal, -inf below."""
+        self._check(self.L.bpmax_tri_fill_random(self.h, n, batch, seed, lo, hi, dev))
+
     def r0_tri_batch(self, n: int, batch: int, A: int, B: int, C: int):
         self._check(self.L.bpmax_r0_tri_batch_device(self.h, n, batch, A, B, C))
 
@@ -235,8 +244,15 @@ class Engine:
         """Capture and replay the pipelined schedule as a HIP graph (default off)."""
         self._check(self.L.bpmax_ctx_set_graphs(self.h, 1 if on else 0))
 
-    def set_timing(self, on: bool):
-        self._check(self.L.bpmax_ctx_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on: bool, classes=None):
+        """HIP events around every launch (or only those of the named kernel classes)."""
+        if classes is None or not on:
+            self._check(self.L.bpmax_ctx_set_timing(self.h, 1 if on else 0))
+        else:
+            mask = 0
+            for c in classes:
+                mask |= 1 << KERNEL_CLASSES.index(c)
+            self._check(self.L.bpmax_ctx_set_timing_mask(self.h, mask))
 
     def reset_stats(self):
         self._check(self.L.bpmax_ctx_reset_stats(self.h))

@@ -144,6 +144,7 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
         delete c;
         return BPMAX_EDEVICE;
     }
+    if (const char *e = getenv("BPMAX_OUTER_SPLIT")) c->outer_split = std::max(0, atoi(e));
     *out = c;
     return BPMAX_OK;
 }
@@ -190,6 +191,15 @@ extern "C" int bpmax_ctx_set_timing(bpmax_ctx *ctx, int enable)
 {
     if (!ctx) return BPMAX_EINVAL;
     ctx->timing = enable != 0;
+    ctx->timing_mask = ~0u;
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_ctx_set_timing_mask(bpmax_ctx *ctx, unsigned mask)
+{
+    if (!ctx) return BPMAX_EINVAL;
+    ctx->timing = mask != 0;
+    ctx->timing_mask = mask;
     return BPMAX_OK;
 }
 
@@ -308,12 +318,14 @@ static int pick_chunk(long long total_steps, long long tiles)
 // Build the per-phase launch list for a set of pairs (descs already laid out), one group
 // per storage tile size T:
 //   S2 by logical block diagonal (cross-tile launch, then closure launch);
-//   for every outer diagonal d1: one R0 launch per storage block diagonal D (all storage
-//   tiles (I,I+D) of every triangle on d1), then the inner closure by logical block
-//   diagonal DL (cross-tile launch, closure launch).
-// Every (d1, D) / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
-// runner can attach its events to each.
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S)
+//   for every outer diagonal d1: R0 launches over ranges of storage block diagonals (by
+//   default one launch per storage diagonal D: all storage tiles (I,I+D) of every triangle
+//   on d1; outer_split > 0 merges them into that many ranges), then the inner closure by
+//   logical block diagonal DL (cross-tile launch, closure launch).
+// Every range / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
+// runner can attach its events to each.  An R0 launch's `diag` is the last storage
+// diagonal of its range.
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split)
 {
     S.items.clear();
     S.launches.clear();
@@ -323,7 +335,7 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S)
         for (size_t k = 0; k < S.groups.size(); ++k)
             if (S.groups[k].T == p.T) g = (int)k;
         if (g < 0) {
-            S.groups.push_back(GroupInfo{p.T, p.nbl / p.nb, 0, 0, 0});
+            S.groups.push_back(GroupInfo{p.T, p.nbl / p.nb, 0, 0, 0, {}});
             g = (int)S.groups.size() - 1;
         }
         GroupInfo &G = S.groups[g];
@@ -360,16 +372,26 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S)
             }
             emit(BPMAX_K_INNER_INTRA, g, true, f, 0, -1, D);
         }
+        // R0 launches per outer diagonal: storage diagonals in `split` contiguous ranges
+        const int split = outer_split <= 0 ? G.maxnb : std::min(outer_split, G.maxnb);
+        std::vector<int> range_end(split);
+        S.groups[g].out_end.assign(G.maxnb, 0);
+        for (int r = 0, Ds = 0; r < split; ++r) {
+            range_end[r] = (int)((long long)G.maxnb * (r + 1) / split) - 1;
+            for (; Ds <= range_end[r]; ++Ds) S.groups[g].out_end[Ds] = range_end[r];
+        }
         // F table, by outer diagonal
         for (int d1 = 0; d1 < G.maxM; ++d1) {
-            for (int Ds = 0; Ds < G.maxnb; ++Ds) {
+            for (int r = 0; r < split; ++r) {
+                const int Da = r == 0 ? 0 : range_end[r - 1] + 1, Db = range_end[r];
                 size_t f = S.items.size();
                 long long tot = 0;
                 double upd = 0;
                 for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].M <= d1 || P[p].nb <= Ds) continue;
-                    tot += (long long)(P[p].M - d1) * (P[p].nb - Ds) * d1 * (Ds + 1);
-                    if (Ds == 0) {
+                    if (P[p].T != T || P[p].M <= d1) continue;
+                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
+                        tot += (long long)(P[p].M - d1) * (P[p].nb - Ds) * d1 * (Ds + 1);
+                    if (r == 0) {
                         const double n = P[p].N;
                         const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
                         upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
@@ -377,13 +399,14 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S)
                 }
                 const int ch = pick_chunk(tot, 0);
                 for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].M <= d1 || P[p].nb <= Ds) continue;
-                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                        for (int I = 0; I + Ds < P[p].nb; ++I)
-                            chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + Ds, 0, 0, 0}, d1 * (Ds + 1), ch,
-                                    true);
+                    if (P[p].T != T || P[p].M <= d1) continue;
+                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
+                        for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                            for (int I = 0; I + Ds < P[p].nb; ++I)
+                                chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + Ds, 0, 0, 0}, d1 * (Ds + 1),
+                                        ch, true);
                 }
-                emit(BPMAX_K_OUTER, g, false, f, upd, d1, Ds);
+                emit(BPMAX_K_OUTER, g, false, f, upd, d1, Db);
             }
             for (int D = 0; D < G.maxnbl; ++D) {
                 if (D > 0) {
@@ -510,7 +533,8 @@ void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<W
 static int timed(bpmax_ctx *ctx, hipStream_t st, int kclass, double updates, const std::function<hipError_t()> &fn)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ctx->timing) {
+    const bool on = ctx->timing && ((ctx->timing_mask >> kclass) & 1u);
+    if (on) {
         while (ctx->ev_pool.size() < ctx->ev_used + 2) {
             hipEvent_t e;
             HIPCHK(ctx, hipEventCreate(&e));
@@ -525,7 +549,7 @@ static int timed(bpmax_ctx *ctx, hipStream_t st, int kclass, double updates, con
         ctx->err = std::string("kernel launch (") + kClassNames[kclass] + "): " + hipGetErrorString(e);
         return BPMAX_EDEVICE;
     }
-    if (ctx->timing) {
+    if (on) {
         HIPCHK(ctx, hipEventRecord(e1, st));
         ctx->pending.push_back(PendingTiming{kclass, updates, e0, e1});
     }
@@ -648,7 +672,7 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
             if ((rc = run_launch(ctx, sO, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
             HIPCHK(ctx, hipEventRecord(ev_out(L.group, L.d1, L.diag), sO));
         } else {
-            const int need = std::min((L.diag + G.rs - 1) / G.rs, G.maxnb - 1);
+            const int need = G.out_end[std::min((L.diag + G.rs - 1) / G.rs, G.maxnb - 1)];
             if (need > out_wait) {
                 HIPCHK(ctx, hipStreamWaitEvent(sI, ev_out(L.group, L.d1, need), 0));
                 out_wait = need;
@@ -781,7 +805,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     std::sort(Ts.begin(), Ts.end());
 
     // The schedule is rebuilt only when the shapes or the schedule settings change.
-    std::vector<long long> key = {ctx->schedule, ctx->task_chunk, npairs};
+    std::vector<long long> key = {ctx->schedule, ctx->task_chunk, ctx->outer_split, npairs};
     for (auto &d : P) {
         key.push_back(d.M);
         key.push_back(d.N);
@@ -803,7 +827,8 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
                 S.items.insert(S.items.end(), tl.begin(), tl.end());
             }
         } else {
-            build_schedule(P, S);
+            // one stream: a single R0 launch per outer diagonal (nothing to overlap with)
+            build_schedule(P, S, ctx->schedule == BPMAX_SCHED_LAUNCH ? 1 : ctx->outer_split);
         }
         ctx->sched_key = key;
     }
@@ -1018,6 +1043,19 @@ extern "C" int bpmax_tri_upload(bpmax_ctx *ctx, int n, int batch, const float *h
     if ((rc = bpx_grow(ctx, ctx->out, bytes)) != BPMAX_OK) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->out.p, host_dense, bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, launch_dense_to_tiled(ctx->stream, n, nb, T, batch, (const float *)ctx->out.p, dev_tiled));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_tri_fill_random(bpmax_ctx *ctx, int n, int batch, unsigned long long seed, int lo, int hi,
+                                     float *dev_tiled)
+{
+    if (!ctx || n < 1 || n >= (1 << 21) || batch < 1 || batch > 65535 || !dev_tiled || lo < 0 || hi < lo ||
+        hi >= (1 << 22))
+        return BPMAX_EINVAL;
+    const int T = 64, nb = (n + T - 1) / T;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_tri_random(ctx->stream, n, nb, T, batch, seed, lo, hi, dev_tiled));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return BPMAX_OK;
 }

@@ -29,6 +29,7 @@ struct Launch {
 struct GroupInfo {
     int T, rs;           // storage tile, logical tiles per storage tile side
     int maxM, maxnb, maxnbl;
+    std::vector<int> out_end; // storage diagonal -> last diagonal of the R0 launch holding it
 };
 
 struct Schedule {
@@ -54,7 +55,7 @@ struct Server; // continuous-batching server (bpmax_server.cpp)
 int choose_tile(int N);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S);
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split);
 void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
                  std::vector<int> &flags);
 
@@ -74,12 +75,15 @@ struct bpmax_ctx {
     void *items_on_device = nullptr;    // meta buffer that already holds sched.items
     std::vector<char> staging;
     // pipelined schedule: second stream, dependency events, cached graph
-    hipStream_t stream2 = nullptr;
+    hipStream_t stream2 = nullptr;       // inner closure
+    int outer_split = 2;                 // R0 launches per outer diagonal, pipelined schedule
+                                         // (0: one per storage diagonal)
     std::vector<hipEvent_t> sync_ev;
     bool graphs = false;
     hipGraphExec_t graph = nullptr;
     std::vector<long long> graph_key;
     bool timing = false;
+    unsigned timing_mask = ~0u;          // kernel classes timed while timing is on
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<bpx::PendingTiming> pending;

@@ -272,7 +272,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         }
         *(float4 *)(Xs + a * L + c4) = x;
         *(float4 *)(Sa + a * L + c4) = va[p];               // S2 tile (I,I)
-        *(float4 *)(Sb + a * L + c4) = diag ? va[p] : vb[p]; // S2 tile (J,J)
+        *(float4 *)(Sb + a * L + c4) = pick(diag, va[p], vb[p]); // S2 tile (J,J)
     }
     if (tid < LT) {
         below[tid] = bl;
@@ -706,6 +706,32 @@ __global__ void k_dense_to_tiled(int n, int nb, int T, const float *__restrict__
         tile_at(dst + m * tstride, r / T, c / T, nb, T)[(r % T) * T + c % T] = v;
     }
 }
+// Synthetic R0 operands: cell (m, r, c), r <= c < n, holds lo + h % (hi - lo + 1) with h the
+// splitmix64 hash of seed ^ (m << 42 | r << 21 | c); every other cell (and padding) -inf.
+__host__ __device__ inline unsigned long long splitmix64(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__global__ void k_tri_random(int n, int nb, int T, unsigned long long seed, int lo, int hi, float *__restrict__ dst)
+{
+    const long long m = blockIdx.y;
+    const long long tstride = tri_ll(nb) * T * T;
+    const int r = blockIdx.x;
+    const unsigned span = (unsigned)(hi - lo + 1);
+    for (int c = threadIdx.x; c < nb * T; c += blockDim.x) {
+        if (c / T < r / T) continue;
+        float v = NEG_INF;
+        if (r < n && c < n && r <= c) {
+            const unsigned long long key = ((unsigned long long)m << 42) | ((unsigned long long)r << 21) | (unsigned)c;
+            v = (float)(lo + (int)(splitmix64(seed ^ key) % span));
+        }
+        tile_at(dst + m * tstride, r / T, c / T, nb, T)[(r % T) * T + c % T] = v;
+    }
+}
+
 __global__ void k_tiled_to_dense(int n, int nb, int T, const float *__restrict__ src, float *__restrict__ dst)
 {
     const long long m = blockIdx.y;
@@ -811,6 +837,12 @@ hipError_t launch_r0tri(hipStream_t st, int T, int nb, const int2 *tiles, int nt
 hipError_t launch_dense_to_tiled(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst)
 {
     hipLaunchKernelGGL(k_dense_to_tiled, dim3(nb * T, batch), dim3(256), 0, st, n, nb, T, src, dst);
+    return hipGetLastError();
+}
+hipError_t launch_tri_random(hipStream_t st, int n, int nb, int T, int batch, unsigned long long seed, int lo, int hi,
+                             float *dst)
+{
+    hipLaunchKernelGGL(k_tri_random, dim3(nb * T, batch), dim3(256), 0, st, n, nb, T, seed, lo, hi, dst);
     return hipGetLastError();
 }
 hipError_t launch_tiled_to_dense(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst)

@@ -23,5 +23,7 @@ hipError_t launch_r0tri(hipStream_t st, int T, int nb, const int2 *tiles, int nt
                         const float *B, float *C);
 hipError_t launch_dense_to_tiled(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst);
 hipError_t launch_tiled_to_dense(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst);
+hipError_t launch_tri_random(hipStream_t st, int n, int nb, int T, int batch, unsigned long long seed, int lo, int hi,
+                             float *dst);
 
 } // namespace bpx

@@ -118,6 +118,12 @@ __device__ __forceinline__ void commit_A(float *As, const FetchA<LT> &f, int csh
     if (cshift < 0 && threadIdx.x < LT) As[threadIdx.x] = (f.have_prev && !lastcol_only) ? f.prev : NEG_INF;
 }
 
+__device__ __forceinline__ float sel(bool ok, float v) { return ok ? v : NEG_INF; }
+__device__ __forceinline__ float4 pick(bool first, const float4 &a, const float4 &b)
+{
+    return make_float4(first ? a.x : b.x, first ? a.y : b.y, first ? a.z : b.z, first ? a.w : b.w);
+}
+
 // B operand: Bs[k][c] = X[K*LT + k + rshift][J*LT + c] (rshift 0 or +1); rows of tiles
 // below the diagonal (or past the last tile) are -inf.
 template <int LT>
@@ -149,12 +155,14 @@ template <int LT>
 __device__ __forceinline__ void commit_B(float *Bs, const FetchB<LT> &f)
 {
     constexpr int LD = Tc<LT>::LD, Q = Stage<LT>::Q;
-    const float4 ninf = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
 #pragma unroll
     for (int p = 0; p < Stage<LT>::PER; ++p) {
         const int idx = threadIdx.x + p * kThreads;
         const int k = idx / Q, c4 = (idx % Q) * 4;
-        *(float4 *)(Bs + k * LD + c4) = ((f.have >> p) & 1u) ? f.v[p] : ninf;
+        // select per component: a select of whole float4 values is lowered through scratch
+        const bool ok = (f.have >> p) & 1u;
+        *(float4 *)(Bs + k * LD + c4) = make_float4(sel(ok, f.v[p].x), sel(ok, f.v[p].y), sel(ok, f.v[p].z),
+                                                    sel(ok, f.v[p].w));
     }
 }
 

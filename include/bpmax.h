@@ -134,6 +134,11 @@ int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const float *A_d
  * the host; entries below the diagonal are ignored on upload, -inf on download). */
 int bpmax_tri_upload(bpmax_ctx *ctx, int n, int batch, const float *host_dense, float *dev_tiled);
 int bpmax_tri_download(bpmax_ctx *ctx, int n, int batch, const float *dev_tiled, float *host_dense);
+/* Fills `batch` tiled triangles on the device with synthetic operands: cell (m, r, c),
+ * r <= c < n, gets lo + splitmix64(seed ^ (m<<42 | r<<21 | c)) % (hi-lo+1); every other
+ * cell -inf.  0 <= lo <= hi < 2^22, n < 2^21. */
+int bpmax_tri_fill_random(bpmax_ctx *ctx, int n, int batch, unsigned long long seed, int lo, int hi,
+                          float *dev_tiled);
 void *bpmax_device_alloc(bpmax_ctx *ctx, size_t bytes);
 int bpmax_device_free(bpmax_ctx *ctx, void *p);
 
@@ -141,6 +146,9 @@ int bpmax_device_free(bpmax_ctx *ctx, void *p);
 /* When enabled, every kernel launch is bracketed by HIP events on the context stream and
  * per-kernel-class totals are accumulated (adds host overhead; off by default). */
 int bpmax_ctx_set_timing(bpmax_ctx *ctx, int enable);
+/* Like bpmax_ctx_set_timing, for the kernel classes whose bit (1 << class) is set in
+ * `mask` only (0 disables timing). */
+int bpmax_ctx_set_timing_mask(bpmax_ctx *ctx, unsigned mask);
 /* Kernel classes for bpmax_ctx_kernel_stats. */
 typedef enum {
     BPMAX_K_FILL = 0,

@@ -266,3 +266,29 @@ def test_graph_replays_match_oracle(eng):
         eng.solve(s1[:2], s2[:9], intra, inter)
         eng.set_timing(False)
         eng.synchronize()
+
+
+def _splitmix64(x):
+    with np.errstate(over="ignore"):
+        x = (x + np.uint64(0x9E3779B97F4A7C15))
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def test_tri_fill_random_matches_host_hash(eng):
+    """The synthetic R0 operands are the documented hash of (seed, matrix, row, column)."""
+    n, batch, seed, lo, hi = 70, 3, 5, 0, 200
+    _say(f"tri_fill_random n={n} batch={batch}")
+    dev = eng.tri_alloc(n, batch)
+    try:
+        eng.tri_fill_random(dev, n, batch, seed, lo, hi)
+        got = eng.tri_download(dev, n, batch)
+    finally:
+        eng.tri_free(dev)
+    m, r, c = np.meshgrid(np.arange(batch, dtype=np.uint64), np.arange(n, dtype=np.uint64),
+                          np.arange(n, dtype=np.uint64), indexing="ij")
+    key = (m << np.uint64(42)) | (r << np.uint64(21)) | c
+    want = (lo + (_splitmix64(np.uint64(seed) ^ key) % np.uint64(hi - lo + 1))).astype(np.float32)
+    want[r > c] = -np.inf
+    assert _bits_equal(got, want)
