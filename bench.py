@@ -15,7 +15,10 @@ The JSON line also carries
                   sources) on this host's cores for the same 64x512 pair;
   batched      -- 256 mixed-size pairs (M~U[16,48], N~U[100,400], GC~U[0.3,0.7]) solved
                   in one batched schedule vs the same pairs solved one per call;
-  r0_isolated  -- the R0 max-plus kernel alone on 4096 tiled 512x512 upper triangles.
+  r0_isolated  -- the R0 max-plus kernel alone on 4096 tiled 512x512 upper triangles;
+  serving      -- the continuous-batching server on mixed requests (M log-uniform
+                  [8,64], N log-uniform [64,1024]): saturated pairs/s and p50/p99 latency
+                  under Poisson arrivals, against one solve call per pair.
 """
 import argparse
 import json
@@ -199,6 +202,95 @@ def r0_leg(B, eng, n=512, batch=4096, reps=3):
             eng.tri_free(p)
 
 
+def serving_requests(n, seed=13):
+    """M log-uniform in [8,64], N log-uniform in [64,1024], GC content 0.5."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        M = int(round(np.exp(rng.uniform(np.log(8), np.log(64)))))
+        N = int(round(np.exp(rng.uniform(np.log(64), np.log(1024)))))
+        out.append((gc_sequence(rng, M, 0.5), gc_sequence(rng, N, 0.5)))
+    return out
+
+
+def _pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if v else None
+
+
+def serving_leg(B, n_plain=40, n_sat=160, n_rate=160, fractions=(0.25, 0.5)):
+    """Continuous-batching server: saturated throughput and p50/p99 latency at Poisson
+    arrival rates below saturation, against the plain path (one solve call per pair)."""
+    reqs = serving_requests(max(n_plain, n_sat, n_rate))
+    cells = [B.cell_updates(len(a), len(b)) for a, b in reqs]
+    eng = B.Engine(0)
+    try:
+        log(f"serving: plain per-pair path on {n_plain} requests")
+        eng.solve(*reqs[0], layout=B.LAYOUT_NONE)
+        t0 = time.perf_counter()
+        plain_scores = [eng.solve(a, b, layout=B.LAYOUT_NONE)[1] for a, b in reqs[:n_plain]]
+        t_plain = time.perf_counter() - t0
+        plain_rate = n_plain / t_plain
+        log(f"serving: plain {plain_rate:.1f} pairs/s")
+        eng.server_start()  # budget: 80% of free HBM, admission by table footprint
+        st0 = eng.server_stats()
+
+        def run(schedule):
+            """schedule: arrival offsets (s) per request index; returns (wall, latencies, scores)."""
+            t_start = time.perf_counter()
+            ids = []
+            for k, t_arr in enumerate(schedule):
+                delay = t_start + t_arr - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(eng.server_submit(*reqs[k]))
+            res = []
+            for i in ids:
+                r = eng.server_wait(i, 5000.0)
+                while r is None:  # still queued or running: report progress, bounded
+                    st = eng.server_stats()
+                    log(f"serving: waiting, {st['completed'] - st0['completed']} done of {st['submitted']}, "
+                        f"{st['bytes_in_use'] / 1e9:.1f} GB in use")
+                    if time.perf_counter() - t_start > 240:
+                        raise RuntimeError("serving run exceeded 240 s")
+                    r = eng.server_wait(i, 5000.0)
+                res.append(r)
+            wall = time.perf_counter() - t_start
+            for i in ids:
+                eng.server_release(i)
+            return wall, [r[1] for r in res], [r[0] for r in res]
+
+        wall, lat, sc = run([0.0] * n_sat)
+        sat_rate = n_sat / wall
+        same = bool(np.array_equal(np.array(sc[:n_plain], np.float32), np.array(plain_scores, np.float32)))
+        log(f"serving: saturated {sat_rate:.1f} pairs/s, p50 {_pct(lat, 50):.1f} ms, p99 {_pct(lat, 99):.1f} ms, "
+            f"scores identical to plain: {same}")
+        rates = []
+        rng = np.random.default_rng(17)
+        for f in fractions:
+            lam = f * sat_rate
+            arr = np.cumsum(rng.exponential(1.0 / lam, n_rate))
+            arr -= arr[0]
+            wall_r, lat_r, _ = run(list(arr))
+            rates.append({"offered_pairs_per_s": round(lam, 2), "achieved_pairs_per_s": round(n_rate / wall_r, 2),
+                          "p50_ms": round(_pct(lat_r, 50), 2), "p99_ms": round(_pct(lat_r, 99), 2)})
+            log(f"serving: offered {lam:.1f}/s -> p50 {rates[-1]['p50_ms']} ms, p99 {rates[-1]['p99_ms']} ms")
+        st = eng.server_stats()
+        eng.server_stop()
+        return {"workload": "pair requests, M log-uniform [8,64], N log-uniform [64,1024], GC 0.5, seed 13; "
+                            "admission by table footprint against 80% of free HBM",
+                "plain_per_pair": {"pairs_per_s": round(plain_rate, 2), "requests": n_plain},
+                "saturated": {"pairs_per_s": round(sat_rate, 2), "requests": n_sat,
+                              "gcups": round(sum(cells[:n_sat]) / wall / 1e9, 1),
+                              "p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
+                              "speedup_vs_plain": round(sat_rate / plain_rate, 3)},
+                "poisson": rates, "scores_identical_to_plain": same,
+                "server": {"budget_gb": round(st0["budget"] / 1e9, 1), "batches": st["batches"],
+                           "max_batch_seen": st["max_batch_seen"], "max_inflight_pairs": st["max_inflight_pairs"],
+                           "peak_gb_in_use": round(st["peak_bytes_in_use"] / 1e9, 2)}}
+    finally:
+        eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,10 +324,11 @@ def main():
     M, N = args.M, args.N
     eng = B.Engine(0)
     res = pair_leg(B, eng, M, N, args.steps, args.warmup)
-    batched = r0 = None
+    batched = r0 = serving = None
     if not args.no_extra:
         batched = batched_leg(B, eng)
         r0 = r0_leg(B, eng)
+        serving = serving_leg(B)
 
     cpu = None
     if not args.no_cpu:
@@ -254,7 +347,7 @@ def main():
             "roofline": res["roofline"], "cpu_baseline": cpu,
             "ms_per_step_without_events": round(res["ms_plain"], 3),
             "phases_pipelined": res["phases_pipelined"], "phases_one_stream": res["phases_one_stream"],
-            "batched": batched, "r0_isolated": r0}
+            "batched": batched, "r0_isolated": r0, "serving": serving}
     print(json.dumps(line), flush=True)
     eng.close()
 

@@ -84,6 +84,15 @@ def load_library(path: str = LIB_PATH):
         "bpmax_base_code": (ctypes.c_ubyte, [ctypes.c_char]),
         "bpmax_read_sequence_file": (c_int, [ctypes.c_char_p, ip, c_int]),
         "bpmax_write_checking_dump": (c_int, [ctypes.c_char_p, fp, c_int, c_int, c_int]),
+        "bpmax_server_start": (c_int, [c_void_p, ctypes.POINTER(ServerConfig)]),
+        "bpmax_server_submit": (c_int, [c_void_p, ip, c_int, ip, c_int, ctypes.POINTER(ctypes.c_longlong)]),
+        "bpmax_server_poll": (c_int, [c_void_p, ctypes.c_longlong, fp, ctypes.POINTER(ctypes.c_double)]),
+        "bpmax_server_wait": (c_int, [c_void_p, ctypes.c_longlong, ctypes.c_double, fp,
+                                      ctypes.POINTER(ctypes.c_double)]),
+        "bpmax_server_fetch": (c_int, [c_void_p, ctypes.c_longlong, c_int, fp]),
+        "bpmax_server_release": (c_int, [c_void_p, ctypes.c_longlong]),
+        "bpmax_server_get_stats": (c_int, [c_void_p, ctypes.POINTER(ServerStats)]),
+        "bpmax_server_stop": (c_int, [c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -91,6 +100,21 @@ def load_library(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = L
     return L
+
+
+class ServerConfig(ctypes.Structure):
+    """bpmax_server_config."""
+    _fields_ = [("memory_budget", ctypes.c_size_t), ("max_batch", ctypes.c_int), ("slots", ctypes.c_int),
+                ("queue_capacity", ctypes.c_int), ("keep_tables", ctypes.c_int),
+                ("intra", ctypes.POINTER(ctypes.c_float)), ("inter", ctypes.POINTER(ctypes.c_float))]
+
+
+class ServerStats(ctypes.Structure):
+    """bpmax_server_stats."""
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+                ("batches", ctypes.c_longlong), ("max_batch_seen", ctypes.c_longlong),
+                ("max_inflight_pairs", ctypes.c_longlong), ("budget", ctypes.c_size_t),
+                ("bytes_in_use", ctypes.c_size_t), ("peak_bytes_in_use", ctypes.c_size_t)]
 
 
 def _ip(a):
@@ -267,6 +291,62 @@ class Engine:
                                                       ctypes.byref(upd)))
             out[name] = {"launches": n.value, "ms": ms.value, "updates": upd.value}
         return out
+
+    # -------------------------------------------------------------- serving loop
+    def server_start(self, memory_budget: int = 0, max_batch: int = 0, slots: int = 0, queue_capacity: int = 0,
+                     keep_tables: bool = False, intra=None, inter=None):
+        """Start the continuous-batching server on this context (bpmax_server_start)."""
+        ti = _table(intra, None)
+        te = _table(inter, ti)
+        self._server_tables = (ti, te)  # kept alive while the server runs
+        cfg = ServerConfig(memory_budget, max_batch, slots, queue_capacity, 1 if keep_tables else 0,
+                           None if ti is None else _fp(ti), None if te is None else _fp(te))
+        self._check(self.L.bpmax_server_start(self.h, ctypes.byref(cfg)))
+
+    def server_submit(self, seq1, seq2) -> int:
+        s1 = np.ascontiguousarray(seq1, np.int32)
+        s2 = np.ascontiguousarray(seq2, np.int32)
+        rid = ctypes.c_longlong()
+        self._check(self.L.bpmax_server_submit(self.h, _ip(s1), len(s1), _ip(s2), len(s2), ctypes.byref(rid)))
+        return rid.value
+
+    def server_poll(self, rid: int):
+        """(score, latency_ms), or None while the request is queued or running."""
+        sc = ctypes.c_float()
+        lat = ctypes.c_double()
+        rc = self.L.bpmax_server_poll(self.h, rid, ctypes.byref(sc), ctypes.byref(lat))
+        if rc == ENOTREADY:
+            return None
+        self._check(rc)
+        return float(sc.value), float(lat.value)
+
+    def server_wait(self, rid: int, timeout_ms: float = -1.0):
+        sc = ctypes.c_float()
+        lat = ctypes.c_double()
+        rc = self.L.bpmax_server_wait(self.h, rid, timeout_ms, ctypes.byref(sc), ctypes.byref(lat))
+        if rc == ENOTREADY:
+            return None
+        self._check(rc)
+        return float(sc.value), float(lat.value)
+
+    def server_fetch(self, rid: int, M: int, N: int, layout: int = LAYOUT_PACKED) -> np.ndarray:
+        if layout == LAYOUT_PACKED:
+            out = np.empty((M * (M + 1) // 2) * (N * (N + 1) // 2), np.float32)
+        else:
+            out = np.empty((M, M, N, N), np.float32)
+        self._check(self.L.bpmax_server_fetch(self.h, rid, layout, _fp(out)))
+        return out
+
+    def server_release(self, rid: int):
+        self._check(self.L.bpmax_server_release(self.h, rid))
+
+    def server_stats(self) -> dict:
+        st = ServerStats()
+        self._check(self.L.bpmax_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def server_stop(self):
+        self._check(self.L.bpmax_server_stop(self.h))
 
     def synchronize(self):
         self._check(self.L.bpmax_ctx_synchronize(self.h))

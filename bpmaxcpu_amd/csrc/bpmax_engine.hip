@@ -76,14 +76,6 @@ bool weights_exact(const float *intra, const float *inter, int M, int N, std::st
 } // namespace bpx
 
 // ----------------------------------------------------------------------------- ctx
-#define HIPCHK(ctx, expr)                                                                   \
-    do {                                                                                    \
-        hipError_t _e = (expr);                                                             \
-        if (_e != hipSuccess) {                                                             \
-            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                 \
-            return BPMAX_EDEVICE;                                                           \
-        }                                                                                   \
-    } while (0)
 
 int bpx_grow(bpmax_ctx *ctx, DevBuf &b, size_t bytes)
 {
@@ -305,6 +297,9 @@ static void chunked(std::vector<WorkItem> &v, WorkItem base, int steps, int ch, 
     }
 }
 
+// work items an R0 launch aims for (about 8 per CU)
+constexpr long long kOuterItems = 2048;
+
 static int pick_chunk(long long total_steps, long long tiles)
 {
     // aim for ~8 work items per CU (256 CUs) when the work allows it
@@ -385,26 +380,33 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
             for (int r = 0; r < split; ++r) {
                 const int Da = r == 0 ? 0 : range_end[r - 1] + 1, Db = range_end[r];
                 size_t f = S.items.size();
-                long long tot = 0;
+                long long tiles = 0;
                 double upd = 0;
                 for (size_t p = 0; p < P.size(); ++p) {
                     if (P[p].T != T || P[p].M <= d1) continue;
-                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
-                        tot += (long long)(P[p].M - d1) * (P[p].nb - Ds) * d1 * (Ds + 1);
+                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds) tiles += (long long)(P[p].M - d1) * (P[p].nb - Ds);
                     if (r == 0) {
                         const double n = P[p].N;
                         const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
                         upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
                     }
                 }
-                const int ch = pick_chunk(tot, 0);
+                // Items cover whole k1 values (ck of them, every K of each), ordered triangle,
+                // k1-chunk, tile: workgroups running together then read the same source tiles
+                // (row I of F(i1,k1), column J of F(k1+1,j1)) and share them in L2.
+                const int ck = d1 == 0 ? 1 : (int)std::max<long long>(1, (d1 * tiles + kOuterItems - 1) / kOuterItems);
+                const int nck = d1 == 0 ? 1 : (d1 + ck - 1) / ck;
                 for (size_t p = 0; p < P.size(); ++p) {
                     if (P[p].T != T || P[p].M <= d1) continue;
-                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
-                        for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                            for (int I = 0; I + Ds < P[p].nb; ++I)
-                                chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + Ds, 0, 0, 0}, d1 * (Ds + 1),
-                                        ch, true);
+                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                        for (int c = 0; c < nck; ++c)
+                            for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
+                                for (int I = 0; I + Ds < P[p].nb; ++I) {
+                                    const int nK = Ds + 1;
+                                    const int sb = c * ck * nK, se = std::min(d1, (c + 1) * ck) * nK;
+                                    const int fl = (nck > 1 ? WI_ATOMIC : 0) | (c == 0 ? WI_EPILOGUE : 0);
+                                    S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + Ds, sb, se, fl});
+                                }
                 }
                 emit(BPMAX_K_OUTER, g, false, f, upd, d1, Db);
             }
@@ -697,6 +699,7 @@ static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, 
         gk.push_back((long long)(uintptr_t)b->p);
         gk.push_back((long long)b->cap);
     }
+    gk.push_back((long long)(uintptr_t)arena);
     gk.push_back(fl);
     if (!ctx->graph || gk != ctx->graph_key) {
         if (ctx->graph) HIPCHK(ctx, hipGraphExecDestroy(ctx->graph));
@@ -731,7 +734,7 @@ static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, 
 // On success the F tables of all pairs are resident in ctx->arena and ctx->pairs_host
 // describes them.
 int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms, const int *const *seq2s,
-                  const int *Ns, const float *intra_in, const float *inter_in)
+                  const int *Ns, const float *intra_in, const float *inter_in, bool sync)
 {
     if (npairs <= 0) return BPMAX_EINVAL;
     float wt[50];
@@ -785,10 +788,19 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         P[p] = d;
     }
     int rc;
-    if ((rc = bpx_grow(ctx, ctx->arena, (size_t)fl * sizeof(float))) != BPMAX_OK) return rc;
+    if (ctx->arena_ext) {
+        if ((size_t)fl * sizeof(float) > ctx->arena_ext_bytes) {
+            ctx->err = "pairs need " + std::to_string((size_t)fl * sizeof(float)) + " arena bytes, the slice holds " +
+                       std::to_string(ctx->arena_ext_bytes);
+            return BPMAX_ENOMEM;
+        }
+    } else if ((rc = bpx_grow(ctx, ctx->arena, (size_t)fl * sizeof(float))) != BPMAX_OK) {
+        return rc;
+    }
     if ((rc = bpx_grow(ctx, ctx->codes, (size_t)il * sizeof(int))) != BPMAX_OK) return rc;
 
-    std::vector<int> codes((size_t)il, 4);
+    std::vector<int> &codes = ctx->codes_host;
+    codes.assign((size_t)il, 4);
     for (int p = 0; p < npairs; ++p) {
         memcpy(codes.data() + P[p].a, seq1s[p], sizeof(int) * Ms[p]);
         for (int x = 0; x < Ns[p]; ++x) codes[P[p].b + x] = seq2s[p][Ns[p] - 1 - x];
@@ -849,7 +861,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     const float *dw = (const float *)meta;
     const PairDesc *dP = (const PairDesc *)(meta + wbytes);
     const WorkItem *dI = (const WorkItem *)(meta + wbytes + pbytes);
-    float *arena = (float *)ctx->arena.p;
+    float *arena = bpx_arena(ctx);
     const int *dcodes = (const int *)ctx->codes.p;
 
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -908,22 +920,38 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         for (const Launch &L : S.launches)
             if ((rc = run_launch(ctx, st, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
     }
-    // the host vectors (codes, staging) must outlive the async copies
+    // the host vectors (codes, staging) are context members, alive until the next run
+    if (!sync) return BPMAX_OK;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return bpx_collect_timing(ctx);
 }
 
-int bpx_gather_scores(bpmax_ctx *ctx, float *scores)
+int bpx_gather_scores(bpmax_ctx *ctx, float *scores, bool sync)
 {
     const int n = (int)ctx->pairs_host.size();
     int rc;
     if ((rc = bpx_grow(ctx, ctx->scores, n * sizeof(float))) != BPMAX_OK) return rc;
     const size_t wbytes = 256;
     const PairDesc *dP = (const PairDesc *)((char *)ctx->meta.p + wbytes);
-    HIPCHK(ctx, launch_scores(ctx->stream, dP, n, (const float *)ctx->arena.p, (float *)ctx->scores.p));
+    HIPCHK(ctx, launch_scores(ctx->stream, dP, n, bpx_arena(ctx), (float *)ctx->scores.p));
     HIPCHK(ctx, hipMemcpyAsync(scores, ctx->scores.p, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (sync) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return BPMAX_OK;
+}
+
+float *bpx_arena(bpmax_ctx *ctx) { return ctx->arena_ext ? (float *)ctx->arena_ext : (float *)ctx->arena.p; }
+
+// Arena bytes the pairs need (the layout of bpx_run_pairs).
+size_t bpx_arena_bytes(int npairs, const int *Ms, const int *Ns)
+{
+    auto al = [](long long x) { return (x + 63) / 64 * 64; };
+    long long fl = 0;
+    for (int p = 0; p < npairs; ++p) {
+        const int T = choose_tile(Ns[p]), nb = (Ns[p] + T - 1) / T;
+        const long long ts = tri_ll(nb) * T * T;
+        fl += al(tri_ll(Ms[p]) * ts) + al(ts) + al((long long)Ms[p] * Ms[p]);
+    }
+    return (size_t)fl * sizeof(float);
 }
 
 extern "C" int bpmax_engine_solve(bpmax_ctx *ctx, const int *seq1, int M, const int *seq2, int N,
@@ -939,17 +967,21 @@ extern "C" int bpmax_engine_solve(bpmax_ctx *ctx, const int *seq1, int M, const 
         ctx->err = "unknown layout";
         return BPMAX_EINVAL;
     }
-    int rc = bpx_run_pairs(ctx, 1, &seq1, &M, &seq2, &N, intra, inter);
+    int rc = bpx_run_pairs(ctx, 1, &seq1, &M, &seq2, &N, intra, inter, true);
     if (rc != BPMAX_OK) return rc;
     float score = 0.f;
-    if ((rc = bpx_gather_scores(ctx, &score)) != BPMAX_OK) return rc;
+    if ((rc = bpx_gather_scores(ctx, &score, true)) != BPMAX_OK) return rc;
     if (score_out) *score_out = score;
     if (layout == BPMAX_LAYOUT_NONE) return BPMAX_OK;
-    const PairDesc &P = ctx->pairs_host[0];
-    const size_t bytes = bpmax_layout_bytes(M, N, layout);
+    return bpx_copy_out(ctx, bpx_arena(ctx), ctx->pairs_host[0], layout, F_out);
+}
+
+int bpx_copy_out(bpmax_ctx *ctx, const float *arena, const PairDesc &P, bpmax_layout layout, float *F_out)
+{
+    int rc;
+    const size_t bytes = bpmax_layout_bytes(P.M, P.N, layout);
     if ((rc = bpx_grow(ctx, ctx->out, bytes)) != BPMAX_OK) return rc;
     float *dout = (float *)ctx->out.p;
-    const float *arena = (const float *)ctx->arena.p;
     if (layout == BPMAX_LAYOUT_PACKED_TRI_TRI) {
         rc = timed(ctx, ctx->stream, BPMAX_K_CONVERT, 0, [&] { return launch_to_packed(ctx->stream, arena, P, dout); });
     } else {
@@ -970,9 +1002,9 @@ extern "C" int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *c
                                         const float *inter, float *scores)
 {
     if (!ctx || npairs <= 0 || !seq1s || !Ms || !seq2s || !Ns || !scores) return BPMAX_EINVAL;
-    int rc = bpx_run_pairs(ctx, npairs, seq1s, Ms, seq2s, Ns, intra, inter);
+    int rc = bpx_run_pairs(ctx, npairs, seq1s, Ms, seq2s, Ns, intra, inter, true);
     if (rc != BPMAX_OK) return rc;
-    return bpx_gather_scores(ctx, scores);
+    return bpx_gather_scores(ctx, scores, true);
 }
 
 // ------------------------------------------------------------------ isolated R0 kernel

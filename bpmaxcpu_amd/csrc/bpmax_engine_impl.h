@@ -74,6 +74,9 @@ struct bpmax_ctx {
     std::vector<long long> sched_key;   // (schedule, chunk, shapes) the schedule was built for
     void *items_on_device = nullptr;    // meta buffer that already holds sched.items
     std::vector<char> staging;
+    std::vector<int> codes_host;
+    void *arena_ext = nullptr;           // device arena slice owned by someone else (server)
+    size_t arena_ext_bytes = 0;
     // pipelined schedule: second stream, dependency events, cached graph
     hipStream_t stream2 = nullptr;       // inner closure
     int outer_split = 2;                 // R0 launches per outer diagonal, pipelined schedule
@@ -92,9 +95,27 @@ struct bpmax_ctx {
     bpx::Server *server = nullptr;
 };
 
+// Return BPMAX_EDEVICE from the enclosing function, with the message in ctx->err, when a
+// HIP call fails.
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess) {                                                             \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                 \
+            return BPMAX_EDEVICE;                                                           \
+        }                                                                                   \
+    } while (0)
+
 int bpx_grow(bpmax_ctx *ctx, bpx::DevBuf &b, size_t bytes);
+// Lays out and solves the pairs; with sync = false only enqueues (the context's streams
+// hold the work; the host-side copies live in the context until its next run).
 int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms, const int *const *seq2s,
-                  const int *Ns, const float *intra, const float *inter);
-int bpx_gather_scores(bpmax_ctx *ctx, float *scores);
+                  const int *Ns, const float *intra, const float *inter, bool sync);
+// Scores of the last run into `scores` (pinned host memory when sync = false).
+int bpx_gather_scores(bpmax_ctx *ctx, float *scores, bool sync);
+float *bpx_arena(bpmax_ctx *ctx);
+// F of pair P (resident in `arena`) in `layout` to host F_out, through ctx's stream.
+int bpx_copy_out(bpmax_ctx *ctx, const float *arena, const bpx::PairDesc &P, bpmax_layout layout, float *F_out);
+size_t bpx_arena_bytes(int npairs, const int *Ms, const int *Ns);
 int bpx_collect_timing(bpmax_ctx *ctx);
 void bpx_server_shutdown(bpmax_ctx *ctx);

@@ -47,9 +47,10 @@ template <int T>
 __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
-    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD, R = Tc<T>::R;
+    constexpr int LD = Tc<T>::LD, R = Tc<T>::R;
     float *As = smem, *Bs = smem + T * LD;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    int r0, c0;
+    lane_block<T>(r0, c0);
     const int i1 = it.i1, j1 = it.j1, I = it.I, J = it.J, nb = P.nb, M = P.M;
     float acc[R][R];
 #pragma unroll
@@ -72,7 +73,10 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
         commit_B<T>(Bs, fb);
         __syncthreads();
         if (s + 1 < it.step_end) fetch(s + 1); // in flight while this step computes
-        product<T>(acc, As, Bs, ty, tx);
+        const int K = I + s % nK;
+        int kb, ke;
+        quad_k_range<T>(K == I, K == J, false, kb, ke);
+        product<T>(acc, As, Bs, r0, c0, kb, ke);
     }
 
     const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
@@ -88,7 +92,7 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
         for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                const int lr = ty * R + i, lc = tx * R + j;
+                const int lr = r0 + i, lc = c0 + j;
                 float v = s1 + S2t[lr * T + lc];
                 if (d1 == 0 && I * T + lr == J * T + lc) v = w[25 + 5 * a[i1] + b[I * T + lr]]; // inter base
                 if (d1 >= 4) v = fmaxf(v, Pt[lr * T + lc] + w1);
@@ -103,7 +107,7 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
             for (int i = 0; i < R; ++i)
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
-                    const int o = (ty * R + i) * T + tx * R + j;
+                    const int o = (r0 + i) * T + c0 + j;
                     rb[i][j] = Bt[o];
                     ra[i][j] = At[o];
                 }
@@ -126,7 +130,7 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
                 for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], fmaxf(s1a + cb[i][j], ca[i][j] + s1b));
         }
     }
-    store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, ty, tx, (it.flags & WI_ATOMIC) != 0);
+    store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, r0, c0, (it.flags & WI_ATOMIC) != 0);
 }
 
 // ------------------------------------------------------------------------ inner ext
@@ -142,9 +146,10 @@ template <int T, bool MODE_S>
 __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int TX = Tc<LT>::TX, LD = Tc<LT>::LD, R = Tc<LT>::R;
+    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R;
     float *As = smem, *Bs = smem + LT * LD;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    int r0, c0;
+    lane_block<LT>(r0, c0);
     const int I = it.I, J = it.J, nb = P.nb, D = J - I;
     float *S = arena + P.S2;
     float *X = MODE_S ? S : tri_at(arena, P, it.i1, it.j1);
@@ -155,25 +160,31 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
         for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
     FetchA<LT> fa;
     FetchB<LT> fb;
+    // step s: its operand shifts and which operand is a diagonal tile
     int cshift = 0;
-    bool lastcol = false;
+    bool lastcol = false, adiag = false, bdiag = false;
     auto fetch = [&](int s) {
         if (MODE_S) {
             const int K = I + s;
             cshift = 0;
             lastcol = K == I;
+            adiag = bdiag = false;
             fetch_A<T, LT>(fa, S, nb, I, K, 0);
             fetch_B<T, LT>(fb, S, nb, K, J, 1);
-        } else if (s < D) {
+        } else if (s < D) { // R1: X(I,K) x S2(K,J) shifted down one row
             const int K = I + s;
             cshift = 0;
             lastcol = false;
+            adiag = K == I;
+            bdiag = false;
             fetch_A<T, LT>(fa, X, nb, I, K, 0);
             fetch_B<T, LT>(fb, S, nb, K, J, 1);
-        } else {
+        } else {            // R2: S2(I,K2) shifted right one column x X(K2,J)
             const int K2 = I + 1 + (s - D);
             cshift = -1;
             lastcol = false;
+            adiag = false;
+            bdiag = K2 == J;
             fetch_A<T, LT>(fa, S, nb, I, K2, -1);
             fetch_B<T, LT>(fb, X, nb, K2, J, 0);
         }
@@ -183,11 +194,13 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
         __syncthreads();
         commit_A<LT>(As, fa, cshift, lastcol);
         commit_B<LT>(Bs, fb);
+        int kb, ke;
+        quad_k_range<LT>(adiag, bdiag, lastcol, kb, ke);
         __syncthreads();
         if (s + 1 < it.step_end) fetch(s + 1);
-        product<LT>(acc, As, Bs, ty, tx);
+        product<LT>(acc, As, Bs, r0, c0, kb, ke);
     }
-    store_acc<T, LT>(ltile<T, LT>(X, I, J, nb), acc, ty, tx, true);
+    store_acc<T, LT>(ltile<T, LT>(X, I, J, nb), acc, r0, c0, true);
 }
 
 // ---------------------------------------------------------------------- inner intra
@@ -384,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__
                                                     const int *__restrict__ codes, const float *__restrict__ w)
 {
     __shared__ __attribute__((aligned(16))) float smem[Tc<T>::PROD_FLOATS];
-    const WorkItem it = items[blockIdx.x];
+    const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     outer_body<T>(pairs[it.pair], it, arena, codes, w, smem);
 }
 
@@ -393,7 +406,7 @@ __global__ __launch_bounds__(kThreads) void k_inner_ext(const PairDesc *__restri
                                                         const WorkItem *__restrict__ items, float *__restrict__ arena)
 {
     __shared__ __attribute__((aligned(16))) float smem[Tc<Inner<T>::LT>::PROD_FLOATS];
-    const WorkItem it = items[blockIdx.x];
+    const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     ext_body<T, MODE_S>(pairs[it.pair], it, arena, smem);
 }
 
@@ -404,7 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_inner_intra(const PairDesc *__rest
                                                           const float *__restrict__ w)
 {
     __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
-    const WorkItem it = items[blockIdx.x];
+    const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     intra_body<T, MODE_S>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
 }
 
@@ -660,21 +673,23 @@ __global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restri
                                                     const float *__restrict__ A, const float *__restrict__ B,
                                                     float *__restrict__ C)
 {
-    constexpr int TX = Tc<T>::TX, LD = Tc<T>::LD, R = Tc<T>::R;
+    constexpr int LD = Tc<T>::LD, R = Tc<T>::R;
     __shared__ __attribute__((aligned(16))) float As[T * LD];
     __shared__ __attribute__((aligned(16))) float Bs[T * LD];
-    const int2 IJ = tiles[blockIdx.x];
+    const int w = xcd_work_index(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int2 IJ = tiles[w % gridDim.x];
     const long long tstride = tri_ll(nb) * T * T;
-    const long long m = blockIdx.y;
+    const long long m = w / gridDim.x;
     const float *Am = A + m * tstride, *Bm = B + m * tstride;
     float *Cm = C + m * tstride;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    int r0, c0;
+    lane_block<T>(r0, c0);
     float acc[R][R];
     const float *Ct = tile_at(Cm, IJ.x, IJ.y, nb, T);
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < R; ++j) acc[i][j] = Ct[(ty * R + i) * T + tx * R + j];
+        for (int j = 0; j < R; ++j) acc[i][j] = Ct[(r0 + i) * T + c0 + j];
     FetchA<T> fa;
     FetchB<T> fb;
     fetch_A<T, T>(fa, Am, nb, IJ.x, IJ.x, 0);
@@ -688,9 +703,11 @@ __global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restri
             fetch_A<T, T>(fa, Am, nb, IJ.x, K + 1, 0);
             fetch_B<T, T>(fb, Bm, nb, K + 1, IJ.y, 0);
         }
-        product<T>(acc, As, Bs, ty, tx);
+        int kb, ke;
+        quad_k_range<T>(K == IJ.x, K == IJ.y, false, kb, ke);
+        product<T>(acc, As, Bs, r0, c0, kb, ke);
     }
-    store_acc<T, T>(tile_at(Cm, IJ.x, IJ.y, nb, T), acc, ty, tx, false);
+    store_acc<T, T>(tile_at(Cm, IJ.x, IJ.y, nb, T), acc, r0, c0, false);
 }
 
 // Dense row-major n x n batch <-> tiled triangles (lower cells -> -inf on both sides).

@@ -17,13 +17,52 @@ namespace bpx {
 
 constexpr int kThreads = 256; // every phase body runs on 256 lanes (4 waves)
 
+// Workgroups go to the 8 XCDs round-robin in linear dispatch order and every XCD has its
+// own L2.  Work index of linear block b out of n: XCD x = b % 8 gets the contiguous range
+// starting at x*q + min(x, r) (q = n/8, r = n%8), so neighbouring work items -- the tiles
+// of one triangle, which read the same source tiles -- share one L2.  A bijection on [0,n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_work_index(int b, int n)
+{
+    const int x = b % kXcds, l = b / kXcds, q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + l;
+}
+
+// Product tiles: each of the 4 waves owns one QS x QS quadrant of the LT x LT output tile
+// (wave w: rows (w/2)*QS.., columns (w%2)*QS..), each lane an R x R block of it.  On the
+// diagonal steps of a triangular product a quadrant's k-range shrinks (see quad_k_range).
 template <int LT>
 struct Tc {
     static constexpr int R = LT / 16;   // register tile edge: each lane owns R x R cells
-    static constexpr int TX = 16;       // lanes per tile row
+    static constexpr int QS = LT / 2;   // quadrant edge
+    static constexpr int QL = QS / R;   // lanes per quadrant row
     static constexpr int LD = LT + 4;   // LDS row stride of product operands (16-byte rows)
     static constexpr int PROD_FLOATS = 2 * LT * LD;
 };
+
+// First row r0 and column c0 of this lane's R x R block.
+template <int LT>
+__device__ __forceinline__ void lane_block(int &r0, int &c0)
+{
+    constexpr int R = Tc<LT>::R, QS = Tc<LT>::QS, QL = Tc<LT>::QL;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    r0 = (w >> 1) * QS + (l / QL) * R;
+    c0 = (w & 1) * QS + (l % QL) * R;
+}
+
+// Useful k-range [kb, ke) of this wave's quadrant in one product step:
+//   adiag: A is a diagonal tile (A[r][k] = -inf for k < r): rows >= (w/2)*QS need k >= that;
+//   bdiag: B is a diagonal tile (B[k][c] = -inf for k > c): columns < (w%2+1)*QS need k below;
+//   lastcol: only column LT-1 of A is finite.
+// Multiples of QS; wave-uniform.
+template <int LT>
+__device__ __forceinline__ void quad_k_range(bool adiag, bool bdiag, bool lastcol, int &kb, int &ke)
+{
+    constexpr int QS = Tc<LT>::QS;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    kb = lastcol ? LT - QS : (adiag ? (w >> 1) * QS : 0);
+    ke = bdiag ? ((w & 1) + 1) * QS : LT;
+}
 
 // Storage tile (I,J) (runtime T), for the layout conversions.
 __device__ __forceinline__ const float *tile_at(const float *tri, int I, int J, int nb, int T)
@@ -166,28 +205,39 @@ __device__ __forceinline__ void commit_B(float *Bs, const FetchB<LT> &f)
     }
 }
 
-// acc[i][j] = max(acc[i][j], max_k As[k][ty*R+i] + Bs[k][tx*R+j]) over the whole tile depth:
-// two k per round, v_add_f32 x2 + v_max3_f32 per pair of cell updates.
+// v_max3_f32 as one instruction.  Written out because the compiler regroups chains of
+// maxima into v_max_f32 + v_max3_f32 pairs, 1.75 instead of 1.5 VALU per cell update.
+__device__ __forceinline__ float max3(float a, float b, float c)
+{
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// acc[i][j] = max(acc[i][j], max_{kb<=k<ke} As[k][r0+i] + Bs[k][c0+j]) (kb, ke multiples
+// of QS): two k per round, v_add_f32 x2 + v_max3_f32 per pair of cell updates.
 template <int LT>
 __device__ __forceinline__ void product(float (&acc)[Tc<LT>::R][Tc<LT>::R], const float *As, const float *Bs,
-                                        int ty, int tx)
+                                        int r0, int c0, int kb, int ke)
 {
-    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R;
-#pragma unroll 4
-    for (int k = 0; k < LT; k += 2) {
-        float av0[R], av1[R], bv0[R], bv1[R];
+    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R, QS = Tc<LT>::QS;
+    for (int k0 = kb; k0 < ke; k0 += QS) {
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-            av0[i] = As[k * LD + ty * R + i];
-            av1[i] = As[(k + 1) * LD + ty * R + i];
-            bv0[i] = Bs[k * LD + tx * R + i];
-            bv1[i] = Bs[(k + 1) * LD + tx * R + i];
+        for (int k = k0; k < k0 + QS; k += 2) {
+            float av0[R], av1[R], bv0[R], bv1[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                av0[i] = As[k * LD + r0 + i];
+                av1[i] = As[(k + 1) * LD + r0 + i];
+                bv0[i] = Bs[k * LD + c0 + i];
+                bv1[i] = Bs[(k + 1) * LD + c0 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j)
+                    acc[i][j] = max3(acc[i][j], av0[i] + bv0[j], av1[i] + bv1[j]);
         }
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-#pragma unroll
-            for (int j = 0; j < R; ++j)
-                acc[i][j] = fmaxf(acc[i][j], fmaxf(av0[i] + bv0[j], av1[i] + bv1[j]));
     }
 }
 
@@ -195,13 +245,13 @@ __device__ __forceinline__ void product(float (&acc)[Tc<LT>::R][Tc<LT>::R], cons
 // final value is >= 0, where float order equals the signed-integer order of the bit
 // patterns -- or store them.
 template <int T, int LT>
-__device__ __forceinline__ void store_acc(float *dst, const float (&acc)[Tc<LT>::R][Tc<LT>::R], int ty, int tx,
+__device__ __forceinline__ void store_acc(float *dst, const float (&acc)[Tc<LT>::R][Tc<LT>::R], int r0, int c0,
                                           bool atomic)
 {
     constexpr int R = Tc<LT>::R;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        float *row = dst + (ty * R + i) * T + tx * R;
+        float *row = dst + (r0 + i) * T + c0;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if (atomic) atomicMax((int *)(row + j), __float_as_int(acc[i][j]));

@@ -97,6 +97,50 @@ int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s
                              const int *Ms, const int *const *seq2s, const int *Ns,
                              const float *intra, const float *inter, float *scores);
 
+/* ---- serving loop ---------------------------------------------------------------------- */
+/* Continuous batching over one device arena of `memory_budget` bytes.  Submitted pairs
+ * queue FIFO; a server thread admits the queue head into a batch whenever a batch slot
+ * is free and the head's F footprint (the tiled layout of bpmax_engine_pair_device_bytes,
+ * ~M(M+1)N(N+1)/4 x 4 B) fits a free extent of the arena, launches the batch
+ * asynchronously, and keeps up to `slots` batches in flight at once, so requests that
+ * arrive while others run are admitted as slots and memory free up and their work
+ * overlaps on the device.  Results are bit-identical to bpmax_engine_solve of the same
+ * pair.  No counterpart in the reference (its bpmax() solves one pair per call). */
+typedef struct {
+    size_t memory_budget; /* arena bytes; 0 = 80% of the free device memory at start */
+    int max_batch;        /* pairs per batch (0 = 64)                                  */
+    int slots;            /* batches in flight at once (0 = 3)                         */
+    int queue_capacity;   /* queued requests before submit returns BPMAX_EBUSY (0 = 4096) */
+    int keep_tables;      /* 1: F tables stay resident until bpmax_server_release and can be
+                             read with bpmax_server_fetch; 0: freed when the score is ready */
+    const float *intra;   /* weight tables of every request (NULL: reference table)    */
+    const float *inter;
+} bpmax_server_config;
+
+typedef struct {
+    long long submitted, completed, failed, batches;
+    long long max_batch_seen;     /* largest batch launched                         */
+    long long max_inflight_pairs; /* most pairs on the device at one time           */
+    size_t budget, bytes_in_use, peak_bytes_in_use;
+} bpmax_server_stats;
+
+int bpmax_server_start(bpmax_ctx *ctx, const bpmax_server_config *cfg);
+/* Queue one pair; *id receives its request id.  BPMAX_ETOOBIG if the pair can never fit
+ * the budget, BPMAX_EBUSY if the queue is full, BPMAX_EWEIGHTS as bpmax_engine_solve. */
+int bpmax_server_submit(bpmax_ctx *ctx, const int *seq1, int M, const int *seq2, int N, long long *id);
+/* BPMAX_OK with the score and submit-to-completion latency, BPMAX_ENOTREADY while
+ * queued or running, BPMAX_ENOTFOUND for an unknown or released id. */
+int bpmax_server_poll(bpmax_ctx *ctx, long long id, float *score, double *latency_ms);
+/* As poll, blocking up to timeout_ms (< 0: no limit). */
+int bpmax_server_wait(bpmax_ctx *ctx, long long id, double timeout_ms, float *score, double *latency_ms);
+/* Copy a finished request's F table (keep_tables = 1) in `layout` to F_out. */
+int bpmax_server_fetch(bpmax_ctx *ctx, long long id, bpmax_layout layout, float *F_out);
+/* Forget a finished request (and free its table when it was the last of its batch). */
+int bpmax_server_release(bpmax_ctx *ctx, long long id);
+int bpmax_server_get_stats(bpmax_ctx *ctx, bpmax_server_stats *stats);
+/* Finish every queued request, then stop the server thread and free the arena. */
+int bpmax_server_stop(bpmax_ctx *ctx);
+
 /* ---- drop-in compatibility symbol ----------------------------------------------- */
 /* Exact signature of the reference entry point (src/bpmax.c:129).  Computes on the GPU
  * and writes the caller's 6-level F pointer tree

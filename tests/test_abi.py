@@ -92,3 +92,15 @@ def test_context_without_gpu_fails_cleanly():
         pytest.skip("a GPU is present")
     assert rc == B.EDEVICE
     assert L.bpmax_ctx_create(None, 0) == B.EINVAL
+
+
+def test_server_entry_points_reject_null_context():
+    import ctypes
+    L = B.load_library()
+    rid = ctypes.c_longlong()
+    s = np.zeros(4, np.int32)
+    assert L.bpmax_server_start(None, None) == B.EINVAL
+    assert L.bpmax_server_submit(None, B._ip(s), 4, B._ip(s), 4, ctypes.byref(rid)) == B.EINVAL
+    assert L.bpmax_server_poll(None, 1, None, None) == B.EINVAL
+    assert L.bpmax_server_release(None, 1) == B.EINVAL
+    assert L.bpmax_server_stop(None) == B.EINVAL
