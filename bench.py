@@ -124,8 +124,13 @@ def pair_leg(B, eng, M, N, steps, warmup):
     flops_per_launch = 2.0 * o["updates"] / max(1, o["launches"])
     avg_s = o["ms"] / max(1, o["launches"]) / 1e3
     achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "k_outer_64x512_traffic.json")
+    if (M, N) == (64, 512) and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)["bytes_per_launch"]  # rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_outer<64>: R0 double max-plus + init/P1/R3/R4 (2 flops per cell update)",
                 "launches": o["launches"], "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops_per_launch}

@@ -102,6 +102,17 @@ extern "C" void bpmax(long M, long N, long N_sec, long N_tile, long R, long MR, 
     for (auto &th : pool) th.join();
 }
 
+// The reference compiles its driver (src/bpmax-wrapper.c) as C++ (src/Makefile: CC=icpc,
+// -std=c++11), so a driver built that way calls the C++-mangled name of the prototype at
+// src/bpmax-wrapper.c:51.  Export that name too, so either build links unchanged.
+extern "C" void bpmax_cxx_linkage(long M, long N, long N_sec, long N_tile, long R, long MR, long NR, long P,
+                                  int *seq1, int *seq2, float ******FTable) __asm__("_Z5bpmaxllllllllPiS_PPPPPPf");
+extern "C" void bpmax_cxx_linkage(long M, long N, long N_sec, long N_tile, long R, long MR, long NR, long P,
+                                  int *seq1, int *seq2, float ******FTable)
+{
+    bpmax(M, N, N_sec, N_tile, R, MR, NR, P, seq1, seq2, FTable);
+}
+
 extern "C" int bpmax_read_sequence_file(const char *path, int *out, int cap)
 {
     // FASTA-like record: '>' header lines skipped, letters/digits/'&' kept, ';' ends the

@@ -26,6 +26,8 @@ ORACLE_SO = os.path.join(HERE, "libbpmax_oracle.so")
 REF_RUN = os.path.join(HERE, "_ref", "bpmax_ref_run")
 REF_CLI = os.path.join(HERE, "_ref", "bpmax")
 REF_CHECK = os.path.join(HERE, "_ref", "bpmax.check")
+# the reference's CHECKING driver linked against libbpmax_hip.so (make -C oracle dropin)
+REF_DROPIN_CHECK = os.path.join(HERE, "_ref", "bpmax_dropin.check")
 
 
 def _pair_table(au: float, gc: float, gu: float) -> np.ndarray:

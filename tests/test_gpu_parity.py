@@ -2,6 +2,7 @@
 bit-for-bit.  Every test calls through the C ABI (libbpmax_hip.so via ctypes)."""
 import ctypes
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -292,3 +293,24 @@ def test_tri_fill_random_matches_host_hash(eng):
     want = (lo + (_splitmix64(np.uint64(seed) ^ key) % np.uint64(hi - lo + 1))).astype(np.float32)
     want[r > c] = -np.inf
     assert _bits_equal(got, want)
+
+
+@pytest.mark.parametrize("name,ntile", [("full_5x17_ref.npz", 16), ("full_16x100_ref.npz", 48)])
+def test_reference_driver_linked_to_gpu_library(name, ntile, eng):
+    """The reference's own driver (bpmax-wrapper.c + external_functions.c, CHECKING build,
+    unchanged) linked against libbpmax_hip.so instead of the reference engine: its dump of
+    every F cell equals the dump of the reference-written fixture, line for line."""
+    if not os.path.exists(O.REF_DROPIN_CHECK):
+        pytest.skip("oracle/_ref/bpmax_dropin.check not built (make -C oracle dropin)")
+    fx = G.load_full(os.path.join(G.GOLDEN, name))
+    M, N = len(fx["seq1"]), len(fx["seq2"])
+    _say(f"reference driver via the GPU library, {name}")
+    stdin = " ".join(str(int(x)) for x in list(fx["seq1"]) + list(fx["seq2"])) + "\n"
+    r = subprocess.run([O.REF_DROPIN_CHECK, str(M), str(N), str(ntile), "3", "24"], input=stdin,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-500:]
+    got = [ln[ln.index("FTable("):] for ln in r.stdout.splitlines() if "FTable(" in ln]
+    want = [f"FTable({i1},{j1},{i2},{j2})={v:0.2f}" for (i1, j1, i2, j2), v in
+            zip(((i1, j1, i2, j2) for i1 in range(M) for j1 in range(i1, M)
+                 for i2 in range(N) for j2 in range(i2, N)), fx["F"])]
+    assert got == want
