@@ -204,27 +204,35 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
 }
 
 // ---------------------------------------------------------------------- inner intra
-// In-tile closure of logical tile (I,J) of X (F(i1,j1) or S2): cells are processed by
-// anti-diagonal step s = b + (LT-1-a) (local row a, column b); every dependence of a cell
-// (same row to the left, same column below, P2 at (a+1,b-1)) lies on an earlier step.
-// G = 256/LT lanes share a cell and split its in-tile terms
-//   R1: X[a][b'] + SR[b'+1][b]   b' in [diag ? a : 0, b-1]   SR = S2 tile (J,J)
-//   R2: SL[a][a'-1] + X[a'][b]   a' in [a+1, diag ? b : LT-1] SL = S2 tile (I,I)
-// with U = 32/G independent accumulators (U LDS reads in flight per round trip) and no
-// per-term guard: the last round of a lane may run up to PAD = U*G terms past its range,
-// and every such term is -inf by construction -- past the diagonal the S2 factor is a
-// lower-triangle (-inf) entry, past the tile edge it reads the -inf pad rows of Xs / Sb.
-// Lanes then combine with DPP permutes and the cell's P2 = X[a+1][b-1] + intra(b[r], b[c])
-// (when c-r >= 4, fetched before the reduction).  For S2 on a diagonal tile SL = SR = X
-// itself (both factors of S2's split term lie in the tile); S2 additionally starts every
-// upper cell at 0 (its short-interval value).
-template <int T, bool MODE_S>
+// In-tile closure of logical tile (I,J) of X (F(i1,j1) or S2).  Cell (a,b) (local row a,
+// column b) sits on anti-diagonal s = b + (LT-1-a); every in-tile term of it pairs two
+// cells on earlier anti-diagonals:
+//   R1: X[a][k] + SR[k+1][b]   k in [diag ? a : 0, b-1]    SR = S2 tile (J,J)
+//   R2: SL[a][k-1] + X[k][b]   k in [a+1, diag ? b : LT-1] SL = S2 tile (I,I)
+//   P2: X[a+1][b-1] + intra(b[r], b[c]) when c-r >= 4 (across the tile edge: the row
+//       below / column left / corner, loaded up front)
+// (for S2 on a diagonal tile SL = SR = X itself, and S2 starts every upper cell at 0).
+//
+// Push formulation: each lane owns the 4 cells (q + 8m, b), m = 0..3, of column b =
+// tid % LT (q = tid / LT) and keeps their running maxima in registers.  Step s finalizes
+// anti-diagonal s (owners write max(acc, P2) to LDS), one barrier, then every lane folds
+// in the terms whose newest factor just became final: for each owned cell the one new
+// cell of its row, X[a][a+s-(LT-1)] (R1), and the one new cell of its column,
+// X[b-s+(LT-1)][b] (R2), each with its S2 partner.  A partner that is not final yet still
+// holds a valid lower bound (every stored value is the score of some structure), so the
+// early sum never exceeds the optimum, and the exact pair is added when the later factor
+// finalizes.  Out-of-range factors read -inf pads (negative columns map to pad column
+// LT, rows past LT-1 are the pad rows); indices past a cell's own anti-diagonal only
+// touch cells that are already final, whose registers are no longer stored.  All LDS
+// reads of a step are conflict-free: row factors are broadcasts, column factors and S2
+// partners run along consecutive columns.
+template <int T, bool MODE_S, int NT = kThreads>
 __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
     constexpr int LT = Inner<T>::LT;
     using C = Intra<LT>;
-    constexpr int G = C::G, U = C::U, PAD = C::PAD, L = C::L;
+    constexpr int PAD = C::PAD, L = C::L;
     float *Xs = smem + C::XS, *Sa = smem + C::SA, *Sb = smem + C::SB;
     float *below = smem + C::BELOW, *left = below + LT, *cornerp = left + LT;
     float *wt = smem + C::WT, *wl = smem + C::WL;
@@ -235,14 +243,15 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     float *X = MODE_S ? S : tri_at(arena, P, i1, j1);
     const int tid = threadIdx.x;
 
-    // tiles: every global load issued before any LDS store (float4, unrolled, branch-free)
-    constexpr int Q = LT / 4, PER = LT * Q / kThreads;
+    // tiles: every global load issued before any LDS store (float4, unrolled, branch-free;
+    // lanes past the tile's LT*Q float4 chunks re-read chunk 0 and store nothing)
+    constexpr int Q = LT / 4, CH = LT * Q, PER = (CH + NT - 1) / NT;
     const float *Xt = ltile<T, LT>(X, I, J, nb);
     const float *SLt = ltile<T, LT>(S, I, I, nb), *SRt = ltile<T, LT>(S, J, J, nb);
     float4 vx[PER], va[PER], vb[PER];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int idx = tid + p * kThreads, a = idx / Q, c4 = (idx % Q) * 4;
+        const int idx = (tid + p * NT) % CH, a = idx / Q, c4 = (idx % Q) * 4;
         vx[p] = *(const float4 *)(Xt + a * T + c4);
         va[p] = *(const float4 *)(SLt + a * T + c4);
         vb[p] = *(const float4 *)(SRt + a * T + c4);
@@ -261,20 +270,21 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     }
     const float vw = tid < 25 ? w[tid] : 0.f;
     // -inf pads: columns LT..L-1 of every row, rows LT..LT+PAD-1 of Xs and Sb
-    for (int e = tid; e < (LT + PAD) * (L - LT); e += kThreads) {
+    for (int e = tid; e < (LT + PAD) * (L - LT); e += NT) {
         const int a = e / (L - LT), b = LT + e % (L - LT);
         Xs[a * L + b] = NEG_INF;
         Sb[a * L + b] = NEG_INF;
         if (a < LT) Sa[a * L + b] = NEG_INF;
     }
-    for (int e = tid; e < PAD * LT; e += kThreads) {
+    for (int e = tid; e < PAD * LT; e += NT) {
         const int a = LT + e / LT, b = e % LT;
         Xs[a * L + b] = NEG_INF;
         Sb[a * L + b] = NEG_INF;
     }
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int idx = tid + p * kThreads, a = idx / Q, c4 = (idx % Q) * 4;
+        const int idx = tid + p * NT, a = idx / Q, c4 = (idx % Q) * 4;
+        if (CH % NT != 0 && idx >= CH) break;
         float4 x = vx[p];
         if (MODE_S) { // S2 starts every upper cell at 0, its short-interval value
             const int r = I * LT + a, c = J * LT + c4;
@@ -301,66 +311,95 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     const float *SR = diag && MODE_S ? Xs : Sb;
     __syncthreads();
     // P2 weights of every cell: wt[a][b] = intra(b[I*LT+a], b[J*LT+b])
-    for (int e = tid; e < LT * LT; e += kThreads) wt[e] = wl[5 * cr[e / LT] + cc[e % LT]];
+    for (int e = tid; e < LT * LT; e += NT) wt[e] = wl[5 * cr[e / LT] + cc[e % LT]];
     __syncthreads();
 
-    const int q = tid / G, g = tid % G;
-    for (int s = diag ? LT : 0; s <= 2 * LT - 2; ++s) {
-        const int alo = s < LT - 1 ? LT - 1 - s : 0;
-        const int ahi = s > LT - 1 ? 2 * LT - 2 - s : LT - 1;
-        const int a = alo + q;
-        const bool live = a <= ahi;
-        const int b = a + s - (LT - 1);
-        float vv[U];
+    constexpr int NC = LT * LT / NT, QN = NT / LT; // cells per lane, lanes per column
+    static_assert(NT % LT == 0 && NC * QN == LT, "cell mapping");
+    const int b = tid % LT, q = tid / LT;
+    float acc[NC], wtv[NC];
+    bool p2ok[NC];
 #pragma unroll
-        for (int u = 0; u < U; ++u) vv[u] = NEG_INF;
-        float cur = NEG_INF; // the cell's own value and its P2 candidate, fetched up front
-        if (live) {
-            const int r = I * LT + a, c = J * LT + b;
-            cur = Xs[a * L + b];
-            if (c - r >= 4) {
-                float p;
-                if (a + 1 < LT && b >= 1) p = Xs[(a + 1) * L + b - 1];
-                else if (b >= 1) p = below[b - 1];
-                else if (a + 1 < LT) p = left[a + 1];
-                else p = cornerp[0];
-                cur = fmaxf(cur, p + wt[a * LT + b]);
-            }
-            // R1 terms t in [0, n1): X[a][lo1+t] + SR[lo1+t+1][b]; R2 terms t in [0, n2):
-            // SL[a][a+t] + X[a+1+t][b].  Lane g takes t = g, g+G, ...; both streams advance
-            // together, and a stream past its range reads a -inf pad row instead.
-            const int lo1 = diag ? a : 0;
-            const int n1 = b - lo1;
-            const int n2 = (diag ? b : LT - 1) - a;
-            const float *r1x = Xs + a * L + lo1 + g;
-            const float *r1s = SR + (lo1 + 1 + g) * L + b;
-            const float *r2s = SL + a * L + a + g;
-            const float *r2x = Xs + (a + 1 + g) * L + b;
-            const float *r1s_pad = SR + LT * L + b, *r2x_pad = Xs + LT * L + b;
-            const int nmax = n1 > n2 ? n1 : n2;
-            for (int t = g; t < nmax; t += PAD) {
-                const float *ps = t < n1 ? r1s : r1s_pad;
-                const float *px = t < n2 ? r2x : r2x_pad;
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    vv[u] = fmaxf(vv[u], fmaxf(r1x[u * G] + ps[u * G * L], r2s[u * G] + px[u * G * L]));
-                r1x += PAD;
-                r1s += PAD * L;
-                r2s += PAD;
-                r2x += PAD * L;
-            }
-        }
-        float v = vv[0];
-#pragma unroll
-        for (int u = 1; u < U; ++u) v = fmaxf(v, vv[u]);
-        v = lane_group_max<G>(v);
-        if (live && g == 0) Xs[a * L + b] = fmaxf(v, cur);
-        __syncthreads();
+    for (int m = 0; m < NC; ++m) {
+        const int a = q + m * QN;
+        acc[m] = Xs[a * L + b];
+        wtv[m] = wt[a * LT + b];
+        p2ok[m] = (J * LT + b) - (I * LT + a) >= 4;
     }
-    float *Xo = ltile<T, LT>(X, I, J, nb);
+    // the P2 factor of the owned cell on anti-diagonal s (its neighbour across the tile
+    // edge when the cell lies on the bottom row or left column)
+    auto p2_addr = [&](int s) -> const float * {
+        const int af = b + (LT - 1) - s;
+        if (af < 0 || af >= LT) return cornerp; // no cell of this column on s: unused
+        if (af + 1 < LT && b >= 1) return Xs + (af + 1) * L + b - 1;
+        if (b >= 1) return below + b - 1;
+        if (af + 1 < LT) return left + af + 1;
+        return cornerp;
+    };
+    // finalize the owned cell of anti-diagonal s, if any: max(acc, P2) to LDS
+    auto finalize = [&](int s, float p2) {
+        const int af = b + (LT - 1) - s;
+        if (af < q || af >= LT || (af - q) % QN != 0) return;
+        const int m = (af - q) / QN;
+        float v = NEG_INF, w2 = 0.f;
+        bool ok = false;
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
-        const int idx = tid + p * kThreads, a = idx / Q, c4 = (idx % Q) * 4;
+        for (int mm = 0; mm < NC; ++mm) {
+            v = mm == m ? acc[mm] : v;
+            w2 = mm == m ? wtv[mm] : w2;
+            ok = mm == m ? p2ok[mm] : ok;
+        }
+        if (ok) v = fmaxf(v, p2 + w2);
+        Xs[af * L + b] = v;
+    };
+    // S2 partners of the pushes of anti-diagonal s: constant tiles, read a step ahead --
+    // except for S2's own diagonal tiles, whose partners are cells of the tile itself and
+    // may finalize on the same anti-diagonal as the factor they pair with
+    const bool fresh = MODE_S && diag;
+    float sr[NC], sl[NC];
+    auto partners = [&](int s, float (&r)[NC], float (&l)[NC]) {
+        const int ac = b - s + (LT - 1);
+        const int ksl = ac - 1 < 0 ? 0 : ac - 1;
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            const int a = q + m * QN;
+            const int kr = a + s - (LT - 1);
+            const int ksr = kr + 1 < 0 ? 0 : kr + 1;
+            r[m] = SR[ksr * L + b];
+            l[m] = SL[a * L + ksl];
+        }
+    };
+    const int s0 = diag ? LT - 1 : 0;
+    finalize(s0, *p2_addr(s0));
+    partners(s0, sr, sl);
+    for (int s = s0; s < 2 * LT - 2; ++s) {
+        __syncthreads(); // anti-diagonal s is final
+        // the factors anti-diagonal s just completed: one per owned row, one per column
+        const int ac = b - s + (LT - 1);
+        const float xcol = Xs[(ac < 0 ? 0 : ac) * L + b];
+        float xr[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            const int kr = q + m * QN + s - (LT - 1);
+            xr[m] = Xs[(q + m * QN) * L + (kr < 0 ? LT : kr)]; // negative: -inf pad column
+        }
+        const float p2n = *p2_addr(s + 1); // anti-diagonal s-1: final
+        float srn[NC], sln[NC];
+        if (fresh) partners(s, sr, sl);
+        partners(s + 1, srn, sln);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) acc[m] = max3(acc[m], xr[m] + sr[m], sl[m] + xcol);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            sr[m] = srn[m];
+            sl[m] = sln[m];
+        }
+        finalize(s + 1, p2n);
+    }
+    __syncthreads();
+    float *Xo = ltile<T, LT>(X, I, J, nb);
+    for (int idx = tid; idx < CH; idx += NT) {
+        const int a = idx / Q, c4 = (idx % Q) * 4;
         *(float4 *)(Xo + a * T + c4) = *(const float4 *)(Xs + a * L + c4);
     }
 }
@@ -410,15 +449,19 @@ __global__ __launch_bounds__(kThreads) void k_inner_ext(const PairDesc *__restri
     ext_body<T, MODE_S>(pairs[it.pair], it, arena, smem);
 }
 
+// the in-tile closure runs one cell per lane: 1024 lanes, 4 waves per SIMD to interleave
+// the short dependent chain of each anti-diagonal step
+constexpr int kIntraThreads = 1024;
+
 template <int T, bool MODE_S>
-__global__ __launch_bounds__(kThreads) void k_inner_intra(const PairDesc *__restrict__ pairs,
+__global__ __launch_bounds__(kIntraThreads) void k_inner_intra(const PairDesc *__restrict__ pairs,
                                                           const WorkItem *__restrict__ items,
                                                           float *__restrict__ arena, const int *__restrict__ codes,
                                                           const float *__restrict__ w)
 {
     __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
-    intra_body<T, MODE_S>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
+    intra_body<T, MODE_S, kIntraThreads>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
 }
 
 __global__ __launch_bounds__(kThreads) void k_s1(const PairDesc *__restrict__ pairs, float *__restrict__ arena,
@@ -794,9 +837,9 @@ hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const P
 {
     if (n <= 0) return hipSuccess;
     if (single_strand) {
-        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, true>), dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, true>), dim3(n), dim3(kIntraThreads), 0, st, pairs, items, arena, codes, w));
     } else {
-        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, false>), dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, false>), dim3(n), dim3(kIntraThreads), 0, st, pairs, items, arena, codes, w));
     }
     return hipGetLastError();
 }

@@ -274,12 +274,12 @@ __device__ __forceinline__ float lane_group_max(float v)
     return v;
 }
 
-// LDS layout of the in-tile closure on an LT x LT tile: G lanes per cell, U accumulators
-// per lane, PAD = U*G extra -inf rows after Xs and Sb (see intra_body), then the tile's
-// P2 weights wt[a][b] = intra(b[r], b[c]) and the boundary rows/columns.
+// LDS layout of the in-tile closure on an LT x LT tile: PAD = LT extra -inf rows after Xs
+// and Sb and L - LT = 4 -inf columns per row (see intra_body), then the tile's P2 weights
+// wt[a][b] = intra(b[r], b[c]) and the boundary rows/columns.
 template <int LT>
 struct Intra {
-    static constexpr int G = kThreads / LT, U = 32 / G, PAD = U * G, L = LT + 4;
+    static constexpr int PAD = LT, L = LT + 4;
     static constexpr int XS = 0, SA = (LT + PAD) * L, SB = SA + LT * L, WT = SB + (LT + PAD) * L;
     static constexpr int BELOW = WT + LT * LT, WL = BELOW + 2 * LT + 4, CODES = WL + 28;
     static constexpr int FLOATS = CODES + 2 * LT;
