@@ -84,6 +84,13 @@ def load_library(path: str = LIB_PATH):
         "bpmax_base_code": (ctypes.c_ubyte, [ctypes.c_char]),
         "bpmax_read_sequence_file": (c_int, [ctypes.c_char_p, ip, c_int]),
         "bpmax_write_checking_dump": (c_int, [ctypes.c_char_p, fp, c_int, c_int, c_int]),
+        "bpmax_tune_shape": (c_int, [c_void_p, c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning),
+                                     ctypes.POINTER(ctypes.c_double)]),
+        "bpmax_tune_batch": (c_int, [c_void_p, c_int, ipp, ip, ipp, ip, c_int, ctypes.POINTER(Tuning),
+                                     ctypes.POINTER(ctypes.c_double)]),
+        "bpmax_ctx_set_tuning": (c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
+        "bpmax_ctx_get_tuning": (c_int, [c_void_p, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
+        "bpmax_ctx_set_default_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
         "bpmax_server_start": (c_int, [c_void_p, ctypes.POINTER(ServerConfig)]),
         "bpmax_server_submit": (c_int, [c_void_p, ip, c_int, ip, c_int, ctypes.POINTER(ctypes.c_longlong)]),
         "bpmax_server_poll": (c_int, [c_void_p, ctypes.c_longlong, fp, ctypes.POINTER(ctypes.c_double)]),
@@ -100,6 +107,16 @@ def load_library(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = L
     return L
+
+
+class Tuning(ctypes.Structure):
+    """bpmax_tuning: launch-schedule settings of one problem class."""
+    _fields_ = [("schedule", ctypes.c_int), ("outer_split", ctypes.c_int), ("outer_items", ctypes.c_int),
+                ("graphs", ctypes.c_int), ("intra_wide", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {"schedule": {v: k for k, v in SCHEDULES.items()}[self.schedule], "outer_split": self.outer_split,
+                "outer_items": self.outer_items, "graphs": bool(self.graphs), "intra_wide": self.intra_wide}
 
 
 class ServerConfig(ctypes.Structure):
@@ -291,6 +308,54 @@ class Engine:
                                                       ctypes.byref(upd)))
             out[name] = {"launches": n.value, "ms": ms.value, "updates": upd.value}
         return out
+
+    # ---------------------------------------------------------------- autotuning
+    def tune(self, M: int, N: int, npairs: int = 1, reps: int = 2) -> dict:
+        """Time the candidate launch schedules on synthetic M x N pairs and keep the fastest
+        for the problem class of (M, N, npairs) (bpmax_tune_shape)."""
+        t = Tuning()
+        ms = ctypes.c_double()
+        self._check(self.L.bpmax_tune_shape(self.h, M, N, npairs, reps, ctypes.byref(t), ctypes.byref(ms)))
+        return dict(t.as_dict(), ms=ms.value)
+
+    def tune_batch(self, seqs1, seqs2, reps: int = 1) -> dict:
+        """As tune(), on these pairs (bpmax_tune_batch)."""
+        n = len(seqs1)
+        a1 = [np.ascontiguousarray(x, np.int32) for x in seqs1]
+        a2 = [np.ascontiguousarray(x, np.int32) for x in seqs2]
+        P = ctypes.POINTER(ctypes.c_int)
+        p1 = (P * n)(*[_ip(a) for a in a1])
+        p2 = (P * n)(*[_ip(a) for a in a2])
+        Ms = np.array([len(a) for a in a1], np.int32)
+        Ns = np.array([len(a) for a in a2], np.int32)
+        t = Tuning()
+        ms = ctypes.c_double()
+        self._check(self.L.bpmax_tune_batch(self.h, n, p1, _ip(Ms), p2, _ip(Ns), reps, ctypes.byref(t),
+                                            ctypes.byref(ms)))
+        return dict(t.as_dict(), ms=ms.value)
+
+    def set_tuning(self, M: int, N: int, npairs: int, tuning):
+        """Set the settings of a problem class (dict as returned by tune(); None clears)."""
+        if tuning is None:
+            self._check(self.L.bpmax_ctx_set_tuning(self.h, M, N, npairs, None))
+            return
+        t = Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
+                   1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256))
+        self._check(self.L.bpmax_ctx_set_tuning(self.h, M, N, npairs, ctypes.byref(t)))
+
+    def set_default_tuning(self, tuning: dict):
+        """Settings for problem classes without an entry (bpmax_ctx_set_default_tuning)."""
+        t = Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
+                   1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256))
+        self._check(self.L.bpmax_ctx_set_default_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self, M: int, N: int, npairs: int = 1):
+        t = Tuning()
+        rc = self.L.bpmax_ctx_get_tuning(self.h, M, N, npairs, ctypes.byref(t))
+        if rc == ENOTFOUND:
+            return None
+        self._check(rc)
+        return t.as_dict()
 
     # -------------------------------------------------------------- serving loop
     def server_start(self, memory_budget: int = 0, max_batch: int = 0, slots: int = 0, queue_capacity: int = 0,

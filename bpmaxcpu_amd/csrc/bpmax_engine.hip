@@ -11,6 +11,7 @@
 // schedule, not of the kernels.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -31,6 +32,14 @@ namespace bpx {
 const char *kClassNames[BPMAX_K_NCLASSES] = {"fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri", "tasks"};
 
 int choose_tile(int N) { return N <= 64 ? 32 : 64; }
+
+unsigned long long splitmix64_host(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
 
 size_t pair_device_floats(int M, int N, int T)
 {
@@ -212,6 +221,166 @@ extern "C" int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable)
     return BPMAX_OK;
 }
 
+// ------------------------------------------------------------------------- autotuning
+namespace bpx {
+static int ceil_log2(int x)
+{
+    int k = 0;
+    while ((1 << k) < x) ++k;
+    return k;
+}
+// problem class: M and N rounded up to powers of two, single pair or batch
+long long tuning_class(int M, int N, int npairs)
+{
+    return ((long long)ceil_log2(M) << 16) | ((long long)ceil_log2(N) << 8) | (npairs > 1 ? 1 : 0);
+}
+static bool tuning_valid(const bpmax_tuning *t)
+{
+    return t && (t->schedule == BPMAX_SCHED_PIPE || t->schedule == BPMAX_SCHED_LAUNCH) && t->outer_split >= 0 &&
+           t->outer_items >= 64 && t->intra_wide >= 0;
+}
+} // namespace bpx
+
+extern "C" int bpmax_ctx_set_tuning(bpmax_ctx *ctx, int M, int N, int npairs, const bpmax_tuning *t)
+{
+    if (!ctx || M < 1 || N < 1 || npairs < 1) return BPMAX_EINVAL;
+    const long long cls = tuning_class(M, N, npairs);
+    if (!t) {
+        ctx->tuning.erase(cls);
+        return BPMAX_OK;
+    }
+    if (!tuning_valid(t)) return BPMAX_EINVAL;
+    ctx->tuning[cls] = Tuning{t->schedule, t->outer_split, t->outer_items, t->graphs ? 1 : 0, t->intra_wide};
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *t)
+{
+    if (!ctx || !tuning_valid(t)) return BPMAX_EINVAL;
+    ctx->schedule = t->schedule;
+    ctx->outer_split = t->outer_split;
+    ctx->outer_items = t->outer_items;
+    ctx->graphs = t->graphs != 0;
+    ctx->intra_wide = t->intra_wide;
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_ctx_get_tuning(bpmax_ctx *ctx, int M, int N, int npairs, bpmax_tuning *t)
+{
+    if (!ctx || !t || M < 1 || N < 1 || npairs < 1) return BPMAX_EINVAL;
+    auto it = ctx->tuning.find(tuning_class(M, N, npairs));
+    if (it == ctx->tuning.end()) return BPMAX_ENOTFOUND;
+    *t = bpmax_tuning{it->second.schedule, it->second.outer_split, it->second.outer_items, it->second.graphs,
+                      it->second.intra_wide};
+    return BPMAX_OK;
+}
+
+// Times the candidate launch schedules on the given pairs and keeps the fastest for their
+// class (largest M, largest N, single or batch): one stream or two-stream pipeline, R0
+// launches per outer diagonal, R0 work items per launch, graph replay.  Batches, whose
+// runs cost more, try a shorter list.
+static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const int *Ms, const int *const *p2,
+                      const int *Ns, int reps, bpmax_tuning *best_out, double *best_ms)
+{
+    if (!ctx || npairs < 1 || !p1 || !p2 || !Ms || !Ns || reps < 1) return BPMAX_EINVAL;
+    int mM = 0, mN = 0;
+    for (int p = 0; p < npairs; ++p) {
+        mM = std::max(mM, Ms[p]);
+        mN = std::max(mN, Ns[p]);
+    }
+    std::vector<float> scores(npairs);
+    auto run = [&]() -> int {
+        if (npairs == 1) return bpmax_engine_solve(ctx, p1[0], Ms[0], p2[0], Ns[0], nullptr, nullptr, nullptr,
+                                                   BPMAX_LAYOUT_NONE, scores.data());
+        return bpmax_engine_solve_batch(ctx, npairs, p1, Ms, p2, Ns, nullptr, nullptr, scores.data());
+    };
+    // two rounds: the launch schedule at the default in-tile width threshold, then the
+    // threshold (largest in-tile launch that uses the 1024-lane form) under the best schedule
+    const int w0 = ctx->intra_wide;
+    std::vector<Tuning> cands;
+    if (npairs == 1) {
+        for (int items : {1024, 2048, 4096}) {
+            cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, items, 0, w0});
+            for (int split : {1, 2, 4, 0}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0});
+        }
+        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 2, 2048, 1, w0});
+    } else {
+        cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, 2048, 0, w0});
+        for (int items : {2048, 4096})
+            for (int split : {1, 2, 4}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0});
+    }
+    const long long cls = tuning_class(mM, mN, npairs);
+    const bool had = ctx->tuning.count(cls) != 0;
+    const Tuning saved = had ? ctx->tuning[cls] : Tuning{};
+    Tuning best{};
+    double best_t = 1e300;
+    auto trial = [&](const Tuning &c) -> int {
+        ctx->tuning[cls] = c;
+        int rc = run(); // builds and caches the schedule
+        std::vector<double> ts;
+        for (int r = 0; r < reps && rc == BPMAX_OK; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = run();
+            ts.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (rc != BPMAX_OK) return rc;
+        std::sort(ts.begin(), ts.end());
+        const double med = ts[ts.size() / 2];
+        if (med < best_t) {
+            best_t = med;
+            best = c;
+        }
+        return BPMAX_OK;
+    };
+    int rc = BPMAX_OK;
+    for (const Tuning &c : cands)
+        if ((rc = trial(c)) != BPMAX_OK) break;
+    const Tuning sched_best = best;
+    for (int wide : {0, 256, 1024, 1 << 30}) {
+        if (rc != BPMAX_OK || wide == w0) continue;
+        Tuning c = sched_best;
+        c.intra_wide = wide;
+        rc = trial(c);
+    }
+    if (rc != BPMAX_OK) {
+        if (had) ctx->tuning[cls] = saved;
+        else ctx->tuning.erase(cls);
+        return rc;
+    }
+    ctx->tuning[cls] = best;
+    if (best_out)
+        *best_out = bpmax_tuning{best.schedule, best.outer_split, best.outer_items, best.graphs, best.intra_wide};
+    if (best_ms) *best_ms = best_t;
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_tune_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms,
+                                const int *const *seq2s, const int *Ns, int reps, bpmax_tuning *best,
+                                double *best_ms)
+{
+    return tune_pairs(ctx, npairs, seq1s, Ms, seq2s, Ns, reps, best, best_ms);
+}
+
+// The same on `npairs` synthetic M x N pairs (uniform ACGU, fixed seed).
+extern "C" int bpmax_tune_shape(bpmax_ctx *ctx, int M, int N, int npairs, int reps, bpmax_tuning *best_out,
+                                double *best_ms)
+{
+    if (!ctx || M < 1 || N < 1 || npairs < 1 || reps < 1) return BPMAX_EINVAL;
+    std::vector<std::vector<int>> s1(npairs, std::vector<int>(M)), s2(npairs, std::vector<int>(N));
+    unsigned long long x = 0x5EEDull;
+    for (int p = 0; p < npairs; ++p) {
+        for (int &c : s1[p]) c = (int)((x = splitmix64_host(x)) & 3);
+        for (int &c : s2[p]) c = (int)((x = splitmix64_host(x)) & 3);
+    }
+    std::vector<const int *> p1(npairs), p2(npairs);
+    std::vector<int> Ms(npairs, M), Ns(npairs, N);
+    for (int p = 0; p < npairs; ++p) {
+        p1[p] = s1[p].data();
+        p2[p] = s2[p].data();
+    }
+    return tune_pairs(ctx, npairs, p1.data(), Ms.data(), p2.data(), Ns.data(), reps, best_out, best_ms);
+}
+
 extern "C" int bpmax_ctx_reset_stats(bpmax_ctx *ctx)
 {
     if (!ctx) return BPMAX_EINVAL;
@@ -297,8 +466,6 @@ static void chunked(std::vector<WorkItem> &v, WorkItem base, int steps, int ch, 
     }
 }
 
-// work items an R0 launch aims for (about 8 per CU)
-constexpr long long kOuterItems = 2048;
 
 static int pick_chunk(long long total_steps, long long tiles)
 {
@@ -320,7 +487,7 @@ static int pick_chunk(long long total_steps, long long tiles)
 // Every range / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
 // runner can attach its events to each.  An R0 launch's `diag` is the last storage
 // diagonal of its range.
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split)
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items)
 {
     S.items.clear();
     S.launches.clear();
@@ -394,7 +561,8 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
                 // Items cover whole k1 values (ck of them, every K of each), ordered triangle,
                 // k1-chunk, tile: workgroups running together then read the same source tiles
                 // (row I of F(i1,k1), column J of F(k1+1,j1)) and share them in L2.
-                const int ck = d1 == 0 ? 1 : (int)std::max<long long>(1, (d1 * tiles + kOuterItems - 1) / kOuterItems);
+                const long long want = std::max(64, outer_items);
+                const int ck = d1 == 0 ? 1 : (int)std::max<long long>(1, (d1 * tiles + want - 1) / want);
                 const int nck = d1 == 0 ? 1 : (d1 + ck - 1) / ck;
                 for (size_t p = 0; p < P.size(); ++p) {
                     if (P[p].T != T || P[p].M <= d1) continue;
@@ -593,7 +761,8 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
                      [&] { return launch_inner_ext(st, L.T, L.single_strand, dP, items, L.count, arena); });
     default:
         return timed(ctx, st, L.kind, L.updates, [&] {
-            return launch_inner_intra(st, L.T, L.single_strand, dP, items, L.count, arena, dcodes, dw);
+            return launch_inner_intra(st, L.T, L.single_strand, dP, items, L.count, arena, dcodes, dw,
+                                      L.count <= ctx->eff_intra_wide);
         });
     }
 }
@@ -700,6 +869,7 @@ static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, 
         gk.push_back((long long)b->cap);
     }
     gk.push_back((long long)(uintptr_t)arena);
+    gk.push_back(ctx->eff_intra_wide); // kernel choices that vary between runs of one shape
     gk.push_back(fl);
     if (!ctx->graph || gk != ctx->graph_key) {
         if (ctx->graph) HIPCHK(ctx, hipGraphExecDestroy(ctx->graph));
@@ -816,8 +986,23 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         if (std::find(Ts.begin(), Ts.end(), d.T) == Ts.end()) Ts.push_back(d.T);
     std::sort(Ts.begin(), Ts.end());
 
+    // Launch-schedule settings: the tuned ones of this problem class, else the context's.
+    Tuning eff{ctx->schedule, ctx->outer_split, ctx->outer_items, ctx->graphs ? 1 : 0, ctx->intra_wide};
+    if (ctx->schedule != BPMAX_SCHED_TASKS) {
+        int mM = 0, mN = 0;
+        for (auto &d : P) {
+            mM = std::max(mM, d.M);
+            mN = std::max(mN, d.N);
+        }
+        auto tu = ctx->tuning.find(tuning_class(mM, mN, npairs));
+        if (tu != ctx->tuning.end()) eff = tu->second;
+    }
+    const int sched = eff.schedule;
+    ctx->eff_intra_wide = eff.intra_wide;
+
     // The schedule is rebuilt only when the shapes or the schedule settings change.
-    std::vector<long long> key = {ctx->schedule, ctx->task_chunk, ctx->outer_split, npairs};
+    std::vector<long long> key = {sched, ctx->task_chunk, eff.outer_split, eff.outer_items, npairs};
+    // (intra_wide changes kernel choices, not the schedule: the graph key carries it)
     for (auto &d : P) {
         key.push_back(d.M);
         key.push_back(d.N);
@@ -826,7 +1011,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     if (key != ctx->sched_key) {
         ctx->sched_key.clear();
         ctx->items_on_device = nullptr;
-        if (ctx->schedule == BPMAX_SCHED_TASKS) {
+        if (sched == BPMAX_SCHED_TASKS) {
             S.items.clear();
             S.launches.clear();
             S.groups.clear();
@@ -840,7 +1025,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
             }
         } else {
             // one stream: a single R0 launch per outer diagonal (nothing to overlap with)
-            build_schedule(P, S, ctx->schedule == BPMAX_SCHED_LAUNCH ? 1 : ctx->outer_split);
+            build_schedule(P, S, sched == BPMAX_SCHED_LAUNCH ? 1 : eff.outer_split, eff.outer_items);
         }
         ctx->sched_key = key;
     }
@@ -872,7 +1057,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
                                ctx->stream));
     hipStream_t st = ctx->stream;
 
-    if (ctx->schedule == BPMAX_SCHED_TASKS) {
+    if (sched == BPMAX_SCHED_TASKS) {
         if ((rc = timed(ctx, st, BPMAX_K_FILL, 0, [&] { return launch_fill(st, arena, fl, -INFINITY); })) !=
             BPMAX_OK)
             return rc;
@@ -904,8 +1089,8 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         return bpx_collect_timing(ctx);
     }
 
-    if (ctx->schedule == BPMAX_SCHED_PIPE) {
-        if (ctx->graphs && !ctx->timing)
+    if (sched == BPMAX_SCHED_PIPE) {
+        if (eff.graphs && !ctx->timing)
             rc = launch_pipe_graph(ctx, key, npairs, S, dP, dI, arena, fl, dcodes, dw);
         else
             rc = enqueue_pipe(ctx, npairs, S, dP, dI, arena, fl, dcodes, dw);

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,11 @@ struct Launch {
     int group;           // T group (pairs sharing a storage tile size)
     int d1;              // outer diagonal (-1: S2 phases)
     int diag;            // storage block diagonal (outer) or logical block diagonal (inner)
+};
+
+// launch-schedule settings of one problem class (bpmax_tuning)
+struct Tuning {
+    int schedule, outer_split, outer_items, graphs, intra_wide;
 };
 
 struct GroupInfo {
@@ -55,7 +61,9 @@ struct Server; // continuous-batching server (bpmax_server.cpp)
 int choose_tile(int N);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split);
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items);
+long long tuning_class(int M, int N, int npairs);
+unsigned long long splitmix64_host(unsigned long long x);
 void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
                  std::vector<int> &flags);
 
@@ -81,6 +89,11 @@ struct bpmax_ctx {
     hipStream_t stream2 = nullptr;       // inner closure
     int outer_split = 2;                 // R0 launches per outer diagonal, pipelined schedule
                                          // (0: one per storage diagonal)
+    int outer_items = 2048;              // work items an R0 launch aims for (split-K granularity)
+    int intra_wide = 256;                // in-tile launches of at most this many tiles use the
+                                         // 1024-lane (latency) form, larger ones the 256-lane
+    int eff_intra_wide = 256;            // the current run's threshold
+    std::map<long long, bpx::Tuning> tuning; // tuned settings per problem class
     std::vector<hipEvent_t> sync_ev;
     bool graphs = false;
     hipGraphExec_t graph = nullptr;

@@ -449,19 +449,19 @@ __global__ __launch_bounds__(kThreads) void k_inner_ext(const PairDesc *__restri
     ext_body<T, MODE_S>(pairs[it.pair], it, arena, smem);
 }
 
-// the in-tile closure runs one cell per lane: 1024 lanes, 4 waves per SIMD to interleave
-// the short dependent chain of each anti-diagonal step
-constexpr int kIntraThreads = 1024;
-
-template <int T, bool MODE_S>
-__global__ __launch_bounds__(kIntraThreads) void k_inner_intra(const PairDesc *__restrict__ pairs,
-                                                          const WorkItem *__restrict__ items,
-                                                          float *__restrict__ arena, const int *__restrict__ codes,
-                                                          const float *__restrict__ w)
+// The in-tile closure in two widths: NT = 1024 lanes (one cell per lane, 4 waves per SIMD
+// to interleave the short dependent chain of each anti-diagonal step: lowest latency, for
+// launches of few tiles) or NT = 256 (four cells per lane, the per-step index work shared:
+// highest throughput, for launches of many tiles).
+template <int T, bool MODE_S, int NT>
+__global__ __launch_bounds__(NT) void k_inner_intra(const PairDesc *__restrict__ pairs,
+                                                    const WorkItem *__restrict__ items,
+                                                    float *__restrict__ arena, const int *__restrict__ codes,
+                                                    const float *__restrict__ w)
 {
     __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
-    intra_body<T, MODE_S, kIntraThreads>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
+    intra_body<T, MODE_S, NT>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
 }
 
 __global__ __launch_bounds__(kThreads) void k_s1(const PairDesc *__restrict__ pairs, float *__restrict__ arena,
@@ -833,14 +833,17 @@ hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const Pai
 }
 
 hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
-                              const WorkItem *items, int n, float *arena, const int *codes, const float *w)
+                              const WorkItem *items, int n, float *arena, const int *codes, const float *w, bool wide)
 {
     if (n <= 0) return hipSuccess;
+#define BPX_INTRA(S, NT) \
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, S, NT>), dim3(n), dim3(NT), 0, st, pairs, items, arena, codes, w))
     if (single_strand) {
-        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, true>), dim3(n), dim3(kIntraThreads), 0, st, pairs, items, arena, codes, w));
+        if (wide) { BPX_INTRA(true, 1024); } else { BPX_INTRA(true, kThreads); }
     } else {
-        BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, false>), dim3(n), dim3(kIntraThreads), 0, st, pairs, items, arena, codes, w));
+        if (wide) { BPX_INTRA(false, 1024); } else { BPX_INTRA(false, kThreads); }
     }
+#undef BPX_INTRA
     return hipGetLastError();
 }
 

@@ -9,8 +9,10 @@ hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const Work
                         float *arena, const int *codes, const float *w);
 hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                             const WorkItem *items, int n, float *arena);
+// wide: 1024-lane workgroups (latency) instead of 256 (throughput)
 hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
-                              const WorkItem *items, int n, float *arena, const int *codes, const float *w);
+                              const WorkItem *items, int n, float *arena, const int *codes, const float *w,
+                              bool wide);
 hipError_t launch_tasks(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *tasks, int ntasks, int *head,
                         int *flags, int *err, float *arena, const int *codes, const float *w);
 hipError_t launch_s1(hipStream_t st, const PairDesc *pairs, int npairs, float *arena, const int *codes,

@@ -226,6 +226,35 @@ int bpmax_ctx_reset_stats(bpmax_ctx *ctx);
 #define BPMAX_SCHED_LAUNCH 1
 #define BPMAX_SCHED_PIPE 2
 int bpmax_ctx_set_schedule(bpmax_ctx *ctx, int schedule, int chunk);
+/* ---- autotuning ------------------------------------------------------------------------ */
+/* Launch-schedule settings of one problem class.  A class is (M rounded up to a power of
+ * two, N rounded up to a power of two, single pair or batch); a solve whose class has
+ * settings uses them instead of the context's schedule.  No counterpart in the reference
+ * (its tile and register-tile sizes are command-line parameters). */
+typedef struct {
+    int schedule;    /* BPMAX_SCHED_PIPE or BPMAX_SCHED_LAUNCH                          */
+    int outer_split; /* R0 launches per outer diagonal (pipelined; 0: one per storage
+                        block diagonal)                                                 */
+    int outer_items; /* work items an R0 launch aims for, >= 64 (split-K granularity)   */
+    int graphs;      /* HIP-graph replay of the pipelined schedule                      */
+    int intra_wide;  /* in-tile closure launches of at most this many tiles use 1024-lane
+                        workgroups (one cell per lane: latency), larger ones 256-lane
+                        workgroups (four cells per lane: throughput)                    */
+} bpmax_tuning;
+/* Time every candidate setting on `npairs` synthetic M x N pairs (median of `reps` runs
+ * after a warm-up each), keep the fastest for the class and return it. */
+int bpmax_tune_shape(bpmax_ctx *ctx, int M, int N, int npairs, int reps, bpmax_tuning *best,
+                     double *best_ms);
+/* The same on the caller's own pairs (class: their largest M and N, single or batch). */
+int bpmax_tune_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms,
+                     const int *const *seq2s, const int *Ns, int reps, bpmax_tuning *best,
+                     double *best_ms);
+/* Set (t = NULL: clear) or read the settings of the class of (M, N, npairs);
+ * get returns BPMAX_ENOTFOUND for an untuned class. */
+int bpmax_ctx_set_tuning(bpmax_ctx *ctx, int M, int N, int npairs, const bpmax_tuning *t);
+int bpmax_ctx_get_tuning(bpmax_ctx *ctx, int M, int N, int npairs, bpmax_tuning *t);
+/* The settings for classes without an entry (schedule as bpmax_ctx_set_schedule). */
+int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *t);
 /* Enable or disable (default) HIP-graph capture and replay for BPMAX_SCHED_PIPE; without
  * timing only.  BPMAX_GRAPHS=1 in the environment sets the default. */
 int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);

@@ -104,3 +104,24 @@ def test_server_entry_points_reject_null_context():
     assert L.bpmax_server_poll(None, 1, None, None) == B.EINVAL
     assert L.bpmax_server_release(None, 1) == B.EINVAL
     assert L.bpmax_server_stop(None) == B.EINVAL
+
+
+def test_tuning_entry_points_reject_bad_arguments():
+    L = B.load_library()
+    t = B.Tuning(B.SCHED_PIPE, 2, 2048, 0)
+    assert L.bpmax_ctx_set_tuning(None, 8, 8, 1, ctypes_ref(t)) == B.EINVAL
+    assert L.bpmax_ctx_get_tuning(None, 8, 8, 1, ctypes_ref(t)) == B.EINVAL
+    assert L.bpmax_tune_shape(None, 8, 8, 1, 1, None, None) == B.EINVAL
+
+
+def test_autotune_shape_parser():
+    from bpmaxcpu_amd.autotune import parse_shape
+    assert parse_shape("64x512") == (64, 512, 1)
+    assert parse_shape("48x400x16") == (48, 400, 16)
+    with pytest.raises(ValueError):
+        parse_shape("64")
+
+
+def ctypes_ref(x):
+    import ctypes
+    return ctypes.byref(x)

@@ -30,12 +30,17 @@ def eng():
     e.close()
 
 
-@pytest.fixture(scope="module", params=["pipe", "pipe-eager", "launch", "tasks"])
+@pytest.fixture(scope="module", params=["pipe", "pipe-eager", "launch-narrow", "launch-wide", "tasks"])
 def eng2(request):
     """An engine under each schedule: two-stream pipeline (graph-replayed and eager),
-    per-phase launches on one stream, and the dependency-tile scheduler."""
+    per-phase launches on one stream with the in-tile closure always 256-lane (narrow) or
+    always 1024-lane (wide), and the dependency-tile scheduler."""
     e = B.Engine(0)
-    e.set_schedule(request.param.split("-")[0])
+    sched = request.param.split("-")[0]
+    if sched == "launch":
+        e.set_default_tuning({"schedule": "launch", "outer_split": 1, "outer_items": 2048, "graphs": False,
+                              "intra_wide": 0 if request.param.endswith("narrow") else 1 << 30})
+    e.set_schedule(sched)
     e.set_graphs(request.param == "pipe")
     yield e
     e.close()
@@ -327,3 +332,35 @@ def test_reference_driver_linked_to_gpu_library(name, ntile, eng):
             zip(((i1, j1, i2, j2) for i1 in range(M) for j1 in range(i1, M)
                  for i2 in range(N) for j2 in range(i2, N)), fx["F"])]
     assert got == want
+
+
+def test_autotuned_schedule_is_used_and_exact():
+    """bpmax_tune_shape picks a schedule for the class; solves of that class then run with it
+    and still match the oracle bit for bit; set/get round-trip the table."""
+    eng = B.Engine(0)
+    try:
+        _say("autotune 6x70")
+        best = eng.tune(6, 70, 1, reps=1)
+        assert best["schedule"] in ("pipe", "launch") and best["outer_items"] >= 64 and best["ms"] > 0
+        got = eng.get_tuning(6, 70, 1)
+        assert got == {k: best[k] for k in got}
+        intra, inter = O.table_asym()
+        rng = np.random.default_rng(4)
+        for M, N in [(5, 66), (8, 127)]:  # same class: M <= 8, 65 <= N <= 128
+            s1 = rng.integers(0, 4, M).astype(np.int32)
+            s2 = rng.integers(0, 4, N).astype(np.int32)
+            F, score = eng.solve(s1, s2, intra, inter)
+            Fo, so = O.bpmax(s1, s2, intra, inter)
+            assert score == so and _bits_equal(F, Fo)
+        for setting in [{"schedule": "launch", "outer_split": 1, "outer_items": 64, "graphs": False,
+                         "intra_wide": 0},
+                        {"schedule": "pipe", "outer_split": 0, "outer_items": 4096, "graphs": True,
+                         "intra_wide": 1 << 30}]:
+            eng.set_tuning(6, 70, 1, setting)
+            assert eng.get_tuning(6, 70, 1) == setting
+            F, score = eng.solve(s1, s2, intra, inter)
+            assert score == so and _bits_equal(F, Fo)
+        eng.set_tuning(6, 70, 1, None)
+        assert eng.get_tuning(6, 70, 1) is None
+    finally:
+        eng.close()
