@@ -140,7 +140,8 @@ def pair_leg(B, eng, M, N, steps, warmup):
 
     # one-stream schedule, for the phase breakdown without overlap
     eng.set_tuning(M, N, 1, {"schedule": "launch", "outer_split": 1, "outer_items": tuned["outer_items"],
-                             "graphs": False, "intra_wide": tuned["intra_wide"]})
+                             "graphs": False, "intra_wide": tuned["intra_wide"],
+                             "outer_kgroup": tuned.get("outer_kgroup", 1)})
     eng.set_schedule(B.SCHED_LAUNCH)
     eng.solve(s1, s2, layout=B.LAYOUT_NONE)
     eng.set_timing(True)

@@ -112,11 +112,17 @@ def load_library(path: str = LIB_PATH):
 class Tuning(ctypes.Structure):
     """bpmax_tuning: launch-schedule settings of one problem class."""
     _fields_ = [("schedule", ctypes.c_int), ("outer_split", ctypes.c_int), ("outer_items", ctypes.c_int),
-                ("graphs", ctypes.c_int), ("intra_wide", ctypes.c_int)]
+                ("graphs", ctypes.c_int), ("intra_wide", ctypes.c_int), ("outer_kgroup", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {"schedule": {v: k for k, v in SCHEDULES.items()}[self.schedule], "outer_split": self.outer_split,
-                "outer_items": self.outer_items, "graphs": bool(self.graphs), "intra_wide": self.intra_wide}
+                "outer_items": self.outer_items, "graphs": bool(self.graphs), "intra_wide": self.intra_wide,
+                "outer_kgroup": self.outer_kgroup}
+
+    @staticmethod
+    def from_dict(tuning: dict) -> "Tuning":
+        return Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
+                      1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256), tuning.get("outer_kgroup", 1))
 
 
 class ServerConfig(ctypes.Structure):
@@ -339,14 +345,12 @@ class Engine:
         if tuning is None:
             self._check(self.L.bpmax_ctx_set_tuning(self.h, M, N, npairs, None))
             return
-        t = Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
-                   1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256))
+        t = Tuning.from_dict(tuning)
         self._check(self.L.bpmax_ctx_set_tuning(self.h, M, N, npairs, ctypes.byref(t)))
 
     def set_default_tuning(self, tuning: dict):
         """Settings for problem classes without an entry (bpmax_ctx_set_default_tuning)."""
-        t = Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
-                   1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256))
+        t = Tuning.from_dict(tuning)
         self._check(self.L.bpmax_ctx_set_default_tuning(self.h, ctypes.byref(t)))
 
     def get_tuning(self, M: int, N: int, npairs: int = 1):

@@ -146,6 +146,7 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
         return BPMAX_EDEVICE;
     }
     if (const char *e = getenv("BPMAX_OUTER_SPLIT")) c->outer_split = std::max(0, atoi(e));
+    if (const char *e = getenv("BPMAX_OUTER_KGROUP")) c->outer_kgroup = std::max(0, atoi(e));
     *out = c;
     return BPMAX_OK;
 }
@@ -237,7 +238,7 @@ long long tuning_class(int M, int N, int npairs)
 static bool tuning_valid(const bpmax_tuning *t)
 {
     return t && (t->schedule == BPMAX_SCHED_PIPE || t->schedule == BPMAX_SCHED_LAUNCH) && t->outer_split >= 0 &&
-           t->outer_items >= 64 && t->intra_wide >= 0;
+           t->outer_items >= 64 && t->intra_wide >= 0 && t->outer_kgroup >= 0 && t->outer_kgroup <= 255;
 }
 } // namespace bpx
 
@@ -250,7 +251,8 @@ extern "C" int bpmax_ctx_set_tuning(bpmax_ctx *ctx, int M, int N, int npairs, co
         return BPMAX_OK;
     }
     if (!tuning_valid(t)) return BPMAX_EINVAL;
-    ctx->tuning[cls] = Tuning{t->schedule, t->outer_split, t->outer_items, t->graphs ? 1 : 0, t->intra_wide};
+    ctx->tuning[cls] =
+        Tuning{t->schedule, t->outer_split, t->outer_items, t->graphs ? 1 : 0, t->intra_wide, t->outer_kgroup};
     return BPMAX_OK;
 }
 
@@ -262,6 +264,7 @@ extern "C" int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *
     ctx->outer_items = t->outer_items;
     ctx->graphs = t->graphs != 0;
     ctx->intra_wide = t->intra_wide;
+    ctx->outer_kgroup = t->outer_kgroup;
     return BPMAX_OK;
 }
 
@@ -271,7 +274,7 @@ extern "C" int bpmax_ctx_get_tuning(bpmax_ctx *ctx, int M, int N, int npairs, bp
     auto it = ctx->tuning.find(tuning_class(M, N, npairs));
     if (it == ctx->tuning.end()) return BPMAX_ENOTFOUND;
     *t = bpmax_tuning{it->second.schedule, it->second.outer_split, it->second.outer_items, it->second.graphs,
-                      it->second.intra_wide};
+                      it->second.intra_wide, it->second.outer_kgroup};
     return BPMAX_OK;
 }
 
@@ -296,18 +299,18 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
     };
     // two rounds: the launch schedule at the default in-tile width threshold, then the
     // threshold (largest in-tile launch that uses the 1024-lane form) under the best schedule
-    const int w0 = ctx->intra_wide;
+    const int w0 = ctx->intra_wide, k0 = ctx->outer_kgroup;
     std::vector<Tuning> cands;
     if (npairs == 1) {
         for (int items : {1024, 2048, 4096}) {
-            cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, items, 0, w0});
-            for (int split : {1, 2, 4, 0}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0});
+            cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, items, 0, w0, k0});
+            for (int split : {1, 2, 4, 0}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0});
         }
-        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 2, 2048, 1, w0});
+        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 2, 2048, 1, w0, k0});
     } else {
-        cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, 2048, 0, w0});
+        cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, 2048, 0, w0, k0});
         for (int items : {2048, 4096})
-            for (int split : {1, 2, 4}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0});
+            for (int split : {1, 2, 4}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0});
     }
     const long long cls = tuning_class(mM, mN, npairs);
     const bool had = ctx->tuning.count(cls) != 0;
@@ -336,10 +339,17 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
     for (const Tuning &c : cands)
         if ((rc = trial(c)) != BPMAX_OK) break;
     const Tuning sched_best = best;
-    for (int wide : {0, 256, 1024, 1 << 30}) {
+    for (int wide : {0, 64, 256, 1024, 1 << 30}) {
         if (rc != BPMAX_OK || wide == w0) continue;
         Tuning c = sched_best;
         c.intra_wide = wide;
+        rc = trial(c);
+    }
+    const Tuning wide_best = best;
+    for (int kg : {0, 1, 2}) {
+        if (rc != BPMAX_OK || kg == k0) continue;
+        Tuning c = wide_best;
+        c.outer_kgroup = kg;
         rc = trial(c);
     }
     if (rc != BPMAX_OK) {
@@ -349,7 +359,8 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
     }
     ctx->tuning[cls] = best;
     if (best_out)
-        *best_out = bpmax_tuning{best.schedule, best.outer_split, best.outer_items, best.graphs, best.intra_wide};
+        *best_out = bpmax_tuning{best.schedule, best.outer_split, best.outer_items, best.graphs, best.intra_wide,
+                                 best.outer_kgroup};
     if (best_ms) *best_ms = best_t;
     return BPMAX_OK;
 }
@@ -487,7 +498,7 @@ static int pick_chunk(long long total_steps, long long tiles)
 // Every range / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
 // runner can attach its events to each.  An R0 launch's `diag` is the last storage
 // diagonal of its range.
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items)
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup)
 {
     S.items.clear();
     S.launches.clear();
@@ -547,11 +558,18 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
             for (int r = 0; r < split; ++r) {
                 const int Da = r == 0 ? 0 : range_end[r - 1] + 1, Db = range_end[r];
                 size_t f = S.items.size();
+                // K values per item: a tile's K range split into groups of kg, so that every
+                // item of a (triangle, k1 chunk) takes about one product step per k1 and the
+                // items that run together stay aligned on k1 (each source tile read from L2
+                // by all of its consumers at about the same time)
+                const int kg = d1 == 0 || G.maxnb > 255 ? 0 : outer_kgroup;
+                auto groups = [&](int nK) { return kg > 0 ? (nK + kg - 1) / kg : 1; };
                 long long tiles = 0;
                 double upd = 0;
                 for (size_t p = 0; p < P.size(); ++p) {
                     if (P[p].T != T || P[p].M <= d1) continue;
-                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds) tiles += (long long)(P[p].M - d1) * (P[p].nb - Ds);
+                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
+                        tiles += (long long)(P[p].M - d1) * (P[p].nb - Ds) * groups(Ds + 1);
                     if (r == 0) {
                         const double n = P[p].N;
                         const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
@@ -569,12 +587,15 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
                     for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
                         for (int c = 0; c < nck; ++c)
                             for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
-                                for (int I = 0; I + Ds < P[p].nb; ++I) {
-                                    const int nK = Ds + 1;
-                                    const int sb = c * ck * nK, se = std::min(d1, (c + 1) * ck) * nK;
-                                    const int fl = (nck > 1 ? WI_ATOMIC : 0) | (c == 0 ? WI_EPILOGUE : 0);
-                                    S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + Ds, sb, se, fl});
-                                }
+                                for (int I = 0; I + Ds < P[p].nb; ++I)
+                                    for (int g = 0, ng = groups(Ds + 1); g < ng; ++g) {
+                                        const int koff = kg > 0 ? g * kg : 0;
+                                        const int nK = kg > 0 ? std::min(kg, Ds + 1 - koff) : Ds + 1;
+                                        const int sb = c * ck * nK, se = std::min(d1, (c + 1) * ck) * nK;
+                                        int fl = (nck > 1 || ng > 1 ? WI_ATOMIC : 0) | (c == 0 && g == 0 ? WI_EPILOGUE : 0);
+                                        if (kg > 0) fl |= (koff << WI_KOFF_SHIFT) | (nK << WI_KCNT_SHIFT);
+                                        S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + Ds, sb, se, fl});
+                                    }
                 }
                 emit(BPMAX_K_OUTER, g, false, f, upd, d1, Db);
             }
@@ -808,6 +829,16 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
     hipEvent_t ev_fill = ctx->sync_ev[0], ev_s2 = ctx->sync_ev[1], ev_end = ctx->sync_ev[2];
     auto ev_out = [&](int g, int d1, int D) { return ctx->sync_ev[ob[g] + (size_t)d1 * S.groups[g].maxnb + D]; };
     auto ev_in = [&](int g, int d1, int D) { return ctx->sync_ev[ib[g] + (size_t)d1 * S.groups[g].maxnbl + D]; };
+    // logical diagonals some R0 launch waits for: only those record an event (a record
+    // between two closure launches costs a few microseconds on the critical path)
+    auto in_need = [&](const Launch &L) {
+        const GroupInfo &G = S.groups[L.group];
+        return std::min(G.rs * L.diag + G.rs - 1, G.maxnbl - 1);
+    };
+    std::vector<std::vector<char>> waited(S.groups.size());
+    for (size_t g = 0; g < S.groups.size(); ++g) waited[g].assign(S.groups[g].maxnbl, 0);
+    for (const Launch &L : S.launches)
+        if (L.kind == BPMAX_K_OUTER && L.d1 > 0) waited[L.group][in_need(L)] = 1;
 
     if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0, [&] { return launch_fill(sO, arena, fl, -INFINITY); })) != BPMAX_OK)
         return rc;
@@ -834,7 +865,7 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
                 if (in_wait < 0) HIPCHK(ctx, hipStreamWaitEvent(sO, ev_s2, 0));
                 in_wait = G.maxnbl;
             } else {
-                const int need = std::min(G.rs * L.diag + G.rs - 1, G.maxnbl - 1);
+                const int need = in_need(L);
                 if (need > in_wait) {
                     HIPCHK(ctx, hipStreamWaitEvent(sO, ev_in(L.group, L.d1 - 1, need), 0));
                     in_wait = need;
@@ -849,7 +880,8 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
                 out_wait = need;
             }
             if ((rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
-            if (L.kind == BPMAX_K_INNER_INTRA) HIPCHK(ctx, hipEventRecord(ev_in(L.group, L.d1, L.diag), sI));
+            if (L.kind == BPMAX_K_INNER_INTRA && waited[L.group][L.diag])
+                HIPCHK(ctx, hipEventRecord(ev_in(L.group, L.d1, L.diag), sI));
         }
     }
     HIPCHK(ctx, hipEventRecord(ev_end, sI));
@@ -987,7 +1019,8 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     std::sort(Ts.begin(), Ts.end());
 
     // Launch-schedule settings: the tuned ones of this problem class, else the context's.
-    Tuning eff{ctx->schedule, ctx->outer_split, ctx->outer_items, ctx->graphs ? 1 : 0, ctx->intra_wide};
+    Tuning eff{ctx->schedule, ctx->outer_split, ctx->outer_items, ctx->graphs ? 1 : 0, ctx->intra_wide,
+               ctx->outer_kgroup};
     if (ctx->schedule != BPMAX_SCHED_TASKS) {
         int mM = 0, mN = 0;
         for (auto &d : P) {
@@ -1001,7 +1034,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     ctx->eff_intra_wide = eff.intra_wide;
 
     // The schedule is rebuilt only when the shapes or the schedule settings change.
-    std::vector<long long> key = {sched, ctx->task_chunk, eff.outer_split, eff.outer_items, npairs};
+    std::vector<long long> key = {sched, ctx->task_chunk, eff.outer_split, eff.outer_items, eff.outer_kgroup, npairs};
     // (intra_wide changes kernel choices, not the schedule: the graph key carries it)
     for (auto &d : P) {
         key.push_back(d.M);
@@ -1025,7 +1058,8 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
             }
         } else {
             // one stream: a single R0 launch per outer diagonal (nothing to overlap with)
-            build_schedule(P, S, sched == BPMAX_SCHED_LAUNCH ? 1 : eff.outer_split, eff.outer_items);
+            build_schedule(P, S, sched == BPMAX_SCHED_LAUNCH ? 1 : eff.outer_split, eff.outer_items,
+                           eff.outer_kgroup);
         }
         ctx->sched_key = key;
     }

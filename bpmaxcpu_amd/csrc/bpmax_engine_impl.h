@@ -29,7 +29,7 @@ struct Launch {
 
 // launch-schedule settings of one problem class (bpmax_tuning)
 struct Tuning {
-    int schedule, outer_split, outer_items, graphs, intra_wide;
+    int schedule, outer_split, outer_items, graphs, intra_wide, outer_kgroup;
 };
 
 struct GroupInfo {
@@ -61,7 +61,7 @@ struct Server; // continuous-batching server (bpmax_server.cpp)
 int choose_tile(int N);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items);
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup);
 long long tuning_class(int M, int N, int npairs);
 unsigned long long splitmix64_host(unsigned long long x);
 void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
@@ -90,6 +90,7 @@ struct bpmax_ctx {
     int outer_split = 2;                 // R0 launches per outer diagonal, pipelined schedule
                                          // (0: one per storage diagonal)
     int outer_items = 2048;              // work items an R0 launch aims for (split-K granularity)
+    int outer_kgroup = 1;                // K values per R0 work item (0: every K of the tile)
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
                                          // 1024-lane (latency) form, larger ones the 256-lane
     int eff_intra_wide = 256;            // the current run's threshold

@@ -47,8 +47,11 @@ struct WorkItem {
 enum : int {
     WI_ATOMIC = 1,     // several items write this tile: combine with atomic max
     WI_EPILOGUE = 2,   // this item also adds init/P1/R3/R4 (outer phase)
-    WI_TYPE_SHIFT = 4, // task type (scheduler) in the bits above
+    WI_TYPE_SHIFT = 4, // task type (scheduler), 4 bits
+    WI_KOFF_SHIFT = 8, // outer items: first K of the item's K range minus I, 8 bits
+    WI_KCNT_SHIFT = 16, // outer items: number of K values of the range (0: all of I..J)
 };
+__host__ __device__ inline int wi_type(int flags) { return (flags >> WI_TYPE_SHIFT) & 15; }
 
 enum : int { TASK_S1 = 1, TASK_S2EXT, TASK_S2INTRA, TASK_OUTER, TASK_EXT, TASK_INTRA };
 

@@ -38,50 +38,80 @@ struct Inner {
 };
 
 // ------------------------------------------------------------------------------ outer
-// Output storage tile (I,J) of triangle (i1,j1) over K-steps [step_begin, step_end):
-// step s <-> (k1 = i1 + s/(J-I+1), K = I + s%(J-I+1)):
+// Output storage tile (I,J) of triangle (i1,j1) over K-steps [step_begin, step_end) of the
+// item's K range Kb .. Kb+nK-1 (all of I..J by default):
+// step s <-> (k1 = i1 + s/nK, K = Kb + s%nK):
 //   C[I][J] max= F(i1,k1)[I][K] (x) F(k1+1,j1)[K*T+1 .. K*T+T][J]        (R0, k2+1 shift)
-// With WI_EPILOGUE also S1(i1,j1)+S2 (the inter base case on the diagonal when d1 = 0),
-// P1 and the R3/R4 streams of the same tile.
+// The same steps carry R3 and R4, whose operands are tiles the R0 steps already stage:
+//   K = J: A = F(i1,k1)[I][J]   -> R4  F(i1,k1) + S1(k1+1,j1)
+//   K = I: B = F(k1+1,j1)[I][J] shifted down one row; with row I*T staged above it
+//          (Bs[-1]) the unshifted tile is Bs[r-1]  -> R3  S1(i1,k1) + F(k1+1,j1)
+// so a split of the steps over work items splits R3/R4 with them.  With WI_EPILOGUE also
+// S1(i1,j1)+S2 (the inter base case on the diagonal when d1 = 0) and P1.
+template <int T>
+struct OuterLds {
+    static constexpr int FLOATS = Tc<T>::PROD_FLOATS + Tc<T>::LD; // As, B's extra row, Bs
+};
+
 template <int T>
 __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
-    constexpr int LD = Tc<T>::LD, R = Tc<T>::R;
-    float *As = smem, *Bs = smem + T * LD;
+    constexpr int LD = Tc<T>::LD, R = Tc<T>::R, Q = Stage<T>::Q;
+    float *As = smem, *Bs = smem + (T + 1) * LD;
     int r0, c0;
     lane_block<T>(r0, c0);
     const int i1 = it.i1, j1 = it.j1, I = it.I, J = it.J, nb = P.nb, M = P.M;
+    const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
+    const float *S1 = arena + P.S1;
     float acc[R][R];
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
         for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
 
-    const int nK = J - I + 1;
+    // the item's K values: Kb .. Kb+nK-1 (all of I..J unless the flags name a sub-range)
+    const int kcnt = (it.flags >> WI_KCNT_SHIFT) & 255;
+    const int nK = kcnt ? kcnt : J - I + 1, Kb = I + ((it.flags >> WI_KOFF_SHIFT) & 255);
     FetchA<T> fa;
     FetchB<T> fb;
+    float4 brow = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF); // row I*T of F(k1+1,j1), lanes < Q
     auto fetch = [&](int s) {
-        const int k1 = i1 + s / nK, K = I + s % nK;
+        const int k1 = i1 + s / nK, K = Kb + s % nK;
+        const float *Bt = tri_at(arena, P, k1 + 1, j1);
         fetch_A<T, T>(fa, tri_at(arena, P, i1, k1), nb, I, K, 0);
-        fetch_B<T, T>(fb, tri_at(arena, P, k1 + 1, j1), nb, K, J, 1);
+        fetch_B<T, T>(fb, Bt, nb, K, J, 1);
+        if (K == I && threadIdx.x < Q) brow = *(const float4 *)(Bt + toff + 4 * threadIdx.x);
     };
     if (it.step_begin < it.step_end) fetch(it.step_begin);
     for (int s = it.step_begin; s < it.step_end; ++s) {
+        const int k1 = i1 + s / nK, K = Kb + s % nK;
         __syncthreads();
         commit_A<T>(As, fa, 0, false);
         commit_B<T>(Bs, fb);
+        if (K == I && threadIdx.x < Q) *(float4 *)(Bs - LD + 4 * threadIdx.x) = brow;
         __syncthreads();
         if (s + 1 < it.step_end) fetch(s + 1); // in flight while this step computes
-        const int K = I + s % nK;
+        if (K == J) { // R4: As[c][r] = F(i1,k1)[I*T + r][J*T + c]
+            const float s1b = S1[(k1 + 1) * M + j1];
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], As[(c0 + j) * LD + r0 + i] + s1b);
+        }
+        if (K == I) { // R3: Bs[r-1][c] = F(k1+1,j1)[I*T + r][J*T + c]
+            const float s1a = S1[i1 * M + k1];
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], s1a + Bs[(r0 + i - 1) * LD + c0 + j]);
+        }
         int kb, ke;
         quad_k_range<T>(K == I, K == J, false, kb, ke);
         product<T>(acc, As, Bs, r0, c0, kb, ke);
     }
 
-    const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
     if (it.flags & WI_EPILOGUE) {
-        const float *S1 = arena + P.S1;
         const float *S2t = arena + P.S2 + toff;
         const int *a = codes + P.a, *b = codes + P.b;
         const float s1 = S1[i1 * M + j1];
@@ -98,37 +128,6 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
                 if (d1 >= 4) v = fmaxf(v, Pt[lr * T + lc] + w1);
                 acc[i][j] = fmaxf(acc[i][j], v);
             }
-        // R3 / R4 streams over k1, with the next k1's tiles in flight
-        float ra[R][R], rb[R][R];
-        auto load = [&](int k1) {
-            const float *Bt = tri_at(arena, P, k1 + 1, j1) + toff;
-            const float *At = tri_at(arena, P, i1, k1) + toff;
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const int o = (r0 + i) * T + c0 + j;
-                    rb[i][j] = Bt[o];
-                    ra[i][j] = At[o];
-                }
-        };
-        if (i1 < j1) load(i1);
-        for (int k1 = i1; k1 < j1; ++k1) {
-            const float s1a = S1[i1 * M + k1], s1b = S1[(k1 + 1) * M + j1];
-            float ca[R][R], cb[R][R];
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    ca[i][j] = ra[i][j];
-                    cb[i][j] = rb[i][j];
-                }
-            if (k1 + 1 < j1) load(k1 + 1);
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], fmaxf(s1a + cb[i][j], ca[i][j] + s1b));
-        }
     }
     store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, r0, c0, (it.flags & WI_ATOMIC) != 0);
 }
@@ -222,8 +221,9 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
 // holds a valid lower bound (every stored value is the score of some structure), so the
 // early sum never exceeds the optimum, and the exact pair is added when the later factor
 // finalizes.  Out-of-range factors read -inf pads (negative columns map to pad column
-// LT, rows past LT-1 are the pad rows); indices past a cell's own anti-diagonal only
-// touch cells that are already final, whose registers are no longer stored.  All LDS
+// LT, rows past LT-1 are clamped to the pad row LT); indices past a cell's own
+// anti-diagonal only touch cells that are already final, whose registers are no longer
+// stored (row-end overruns stay inside the LDS layout).  All LDS
 // reads of a step are conflict-free: row factors are broadcasts, column factors and S2
 // partners run along consecutive columns.
 template <int T, bool MODE_S, int NT = kThreads>
@@ -235,7 +235,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     constexpr int PAD = C::PAD, L = C::L;
     float *Xs = smem + C::XS, *Sa = smem + C::SA, *Sb = smem + C::SB;
     float *below = smem + C::BELOW, *left = below + LT, *cornerp = left + LT;
-    float *wt = smem + C::WT, *wl = smem + C::WL;
+    float *wl = smem + C::WL;
     int *cr = (int *)(smem + C::CODES), *cc = cr + LT;
     const int nb = P.nb;
     const bool diag = (I == J);
@@ -269,7 +269,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         vcc = bc[J * LT + tid];
     }
     const float vw = tid < 25 ? w[tid] : 0.f;
-    // -inf pads: columns LT..L-1 of every row, rows LT..LT+PAD-1 of Xs and Sb
+    // -inf pads: columns LT..L-1 of every row, row LT of Xs and Sb
     for (int e = tid; e < (LT + PAD) * (L - LT); e += NT) {
         const int a = e / (L - LT), b = LT + e % (L - LT);
         Xs[a * L + b] = NEG_INF;
@@ -310,9 +310,6 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     const float *SL = diag ? (MODE_S ? Xs : Sb) : Sa;
     const float *SR = diag && MODE_S ? Xs : Sb;
     __syncthreads();
-    // P2 weights of every cell: wt[a][b] = intra(b[I*LT+a], b[J*LT+b])
-    for (int e = tid; e < LT * LT; e += NT) wt[e] = wl[5 * cr[e / LT] + cc[e % LT]];
-    __syncthreads();
 
     constexpr int NC = LT * LT / NT, QN = NT / LT; // cells per lane, lanes per column
     static_assert(NT % LT == 0 && NC * QN == LT, "cell mapping");
@@ -323,7 +320,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     for (int m = 0; m < NC; ++m) {
         const int a = q + m * QN;
         acc[m] = Xs[a * L + b];
-        wtv[m] = wt[a * LT + b];
+        wtv[m] = wl[5 * cr[a] + cc[b]]; // P2 weight intra(b[I*LT+a], b[J*LT+b])
         p2ok[m] = (J * LT + b) - (I * LT + a) >= 4;
     }
     // the P2 factor of the owned cell on anti-diagonal s (its neighbour across the tile
@@ -364,19 +361,58 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         for (int m = 0; m < NC; ++m) {
             const int a = q + m * QN;
             const int kr = a + s - (LT - 1);
-            const int ksr = kr + 1 < 0 ? 0 : kr + 1;
+            const int ksr = clampi(kr + 1, 0, LT);
             r[m] = SR[ksr * L + b];
             l[m] = SL[a * L + ksl];
         }
     };
     const int s0 = diag ? LT - 1 : 0;
+    if constexpr (NC == 1) {
+        // one cell per lane: (a, b) finalizes on anti-diagonal sown, with its P2 factor at
+        // a fixed address; every step is reads, one fused push and a predicated store
+        const int a = q, sown = b + (LT - 1) - a;
+        const float *p2p = p2_addr(sown);
+        float v = acc[0];
+        const float wv = wtv[0];
+        const bool ok = p2ok[0];
+        float *own = Xs + a * L + b;
+        if (s0 == sown) *own = ok ? fmaxf(v, *p2p + wv) : v;
+        float srv = 0.f, slv = 0.f;
+        auto partner = [&](int st, float &r, float &l) {
+            const int kr = a + st - (LT - 1), ac = b - st + (LT - 1);
+            r = SR[clampi(kr + 1, 0, LT) * L + b];
+            l = SL[a * L + (ac - 1 < 0 ? 0 : ac - 1)];
+        };
+        if (!fresh) partner(s0, srv, slv);
+        // the P2 factor read at step s serves the finalize of step s+1 (it lies on
+        // anti-diagonal sown-2, final from step sown-2 on): never waited for on the spot
+        float p2 = *p2p;
+        for (int s = s0; s < 2 * LT - 2; ++s) {
+            __syncthreads(); // anti-diagonal s is final
+            const int kr = a + s - (LT - 1), ac = b - s + (LT - 1);
+            const float xr = Xs[a * L + (kr < 0 ? LT : kr)]; // negative: -inf pad column
+            const float xcol = Xs[clampi(ac, 0, LT) * L + b]; // past the last row: pad row LT
+            const float p2n = *p2p;
+            float srn = 0.f, sln = 0.f;
+            if (fresh) partner(s, srv, slv);
+            else partner(s + 1, srn, sln); // in flight across this step's arithmetic
+            v = max3(v, xr + srv, slv + xcol);
+            const float fin = fmaxf(v, ok ? p2 + wv : NEG_INF);
+            if (s + 1 == sown) *own = fin;
+            p2 = p2n;
+            if (!fresh) {
+                srv = srn;
+                slv = sln;
+            }
+        }
+    } else {
     finalize(s0, *p2_addr(s0));
     partners(s0, sr, sl);
     for (int s = s0; s < 2 * LT - 2; ++s) {
         __syncthreads(); // anti-diagonal s is final
         // the factors anti-diagonal s just completed: one per owned row, one per column
         const int ac = b - s + (LT - 1);
-        const float xcol = Xs[(ac < 0 ? 0 : ac) * L + b];
+        const float xcol = Xs[clampi(ac, 0, LT) * L + b];
         float xr[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
@@ -395,6 +431,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
             sl[m] = sln[m];
         }
         finalize(s + 1, p2n);
+    }
     }
     __syncthreads();
     float *Xo = ltile<T, LT>(X, I, J, nb);
@@ -435,16 +472,22 @@ __global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__
                                                     const WorkItem *__restrict__ items, float *__restrict__ arena,
                                                     const int *__restrict__ codes, const float *__restrict__ w)
 {
-    __shared__ __attribute__((aligned(16))) float smem[Tc<T>::PROD_FLOATS];
+    __shared__ __attribute__((aligned(16))) float smem[OuterLds<T>::FLOATS];
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     outer_body<T>(pairs[it.pair], it, arena, codes, w, smem);
 }
 
+// The closure kernels (cross-tile and in-tile) are the critical path of the pipelined
+// schedule and run beside its R0 launches.  They fit on a CU next to four R0 workgroups
+// (at most 64 VGPRs, i.e. 8 waves per SIMD, and 9 / 15 KB of LDS; the 256-lane in-tile
+// form for F) and raise their wave priority, so they start at once and issue ahead of
+// the R0 waves instead of queueing behind them.
 template <int T, bool MODE_S>
-__global__ __launch_bounds__(kThreads) void k_inner_ext(const PairDesc *__restrict__ pairs,
+__global__ __launch_bounds__(kThreads, 8) void k_inner_ext(const PairDesc *__restrict__ pairs,
                                                         const WorkItem *__restrict__ items, float *__restrict__ arena)
 {
     __shared__ __attribute__((aligned(16))) float smem[Tc<Inner<T>::LT>::PROD_FLOATS];
+    __builtin_amdgcn_s_setprio(3); // latency-critical chain: issue ahead of R0 waves
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     ext_body<T, MODE_S>(pairs[it.pair], it, arena, smem);
 }
@@ -454,12 +497,13 @@ __global__ __launch_bounds__(kThreads) void k_inner_ext(const PairDesc *__restri
 // launches of few tiles) or NT = 256 (four cells per lane, the per-step index work shared:
 // highest throughput, for launches of many tiles).
 template <int T, bool MODE_S, int NT>
-__global__ __launch_bounds__(NT) void k_inner_intra(const PairDesc *__restrict__ pairs,
+__global__ __launch_bounds__(NT, MODE_S ? 1 : 8) void k_inner_intra(const PairDesc *__restrict__ pairs,
                                                     const WorkItem *__restrict__ items,
                                                     float *__restrict__ arena, const int *__restrict__ codes,
                                                     const float *__restrict__ w)
 {
     __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
+    __builtin_amdgcn_s_setprio(3);
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     intra_body<T, MODE_S, NT>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
 }
@@ -528,7 +572,7 @@ __device__ __forceinline__ void wait_storage(Waiter &W, int *flags, const PairDe
 __device__ bool wait_deps(const PairDesc &P, const WorkItem &it, int *flags, const int *err)
 {
     Waiter W{err};
-    const int type = it.flags >> WI_TYPE_SHIFT;
+    const int type = wi_type(it.flags);
     const int I = it.I, J = it.J;
     switch (type) {
     case TASK_S1:
@@ -596,7 +640,7 @@ __device__ bool wait_deps(const PairDesc &P, const WorkItem &it, int *flags, con
 // Thread 0 only, after every lane's stores were released.
 __device__ void signal_done(const PairDesc &P, const WorkItem &it, int *flags)
 {
-    const int type = it.flags >> WI_TYPE_SHIFT;
+    const int type = wi_type(it.flags);
     switch (type) {
     case TASK_S1:
         __hip_atomic_store(flags + P.flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -634,7 +678,7 @@ __global__ __launch_bounds__(kThreads) void k_tasks(const PairDesc *__restrict__
                                                     const int *__restrict__ codes, const float *__restrict__ w)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int SM = cmax(Tc<T>::PROD_FLOATS, cmax(Tc<LT>::PROD_FLOATS, Intra<LT>::FLOATS));
+    constexpr int SM = cmax(OuterLds<T>::FLOATS, cmax(Tc<LT>::PROD_FLOATS, Intra<LT>::FLOATS));
     __shared__ __attribute__((aligned(16))) float smem[SM];
     __shared__ int s_task, s_ok;
     if (threadIdx.x == 0) s_task = atomicAdd(head, 1);
@@ -649,7 +693,7 @@ __global__ __launch_bounds__(kThreads) void k_tasks(const PairDesc *__restrict__
     }
     __syncthreads();
     if (!s_ok) return;
-    switch (it.flags >> WI_TYPE_SHIFT) {
+    switch (wi_type(it.flags)) {
     case TASK_S1: s1_body(P, arena, codes, w); break;
     case TASK_S2EXT: ext_body<T, true>(P, it, arena, smem); break;
     case TASK_S2INTRA: intra_body<T, true>(P, 0, 0, it.I, it.J, arena, codes, w, smem); break;

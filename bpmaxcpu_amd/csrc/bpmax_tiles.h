@@ -274,17 +274,19 @@ __device__ __forceinline__ float lane_group_max(float v)
     return v;
 }
 
-// LDS layout of the in-tile closure on an LT x LT tile: PAD = LT extra -inf rows after Xs
-// and Sb and L - LT = 4 -inf columns per row (see intra_body), then the tile's P2 weights
-// wt[a][b] = intra(b[r], b[c]) and the boundary rows/columns.
+// LDS layout of the in-tile closure on an LT x LT tile: one extra -inf row (row LT) after
+// Xs and Sb and L - LT = 4 -inf columns per row (see intra_body), then the boundary
+// rows/columns, the weight table and the codes.  About 15 KB at LT = 32, so that a
+// 256-lane closure workgroup fits on a CU beside four R0 workgroups.
 template <int LT>
 struct Intra {
-    static constexpr int PAD = LT, L = LT + 4;
-    static constexpr int XS = 0, SA = (LT + PAD) * L, SB = SA + LT * L, WT = SB + (LT + PAD) * L;
-    static constexpr int BELOW = WT + LT * LT, WL = BELOW + 2 * LT + 4, CODES = WL + 28;
+    static constexpr int PAD = 1, L = LT + 4;
+    static constexpr int XS = 0, SA = (LT + PAD) * L, SB = SA + LT * L;
+    static constexpr int BELOW = SB + (LT + PAD) * L, WL = BELOW + 2 * LT + 4, CODES = WL + 28;
     static constexpr int FLOATS = CODES + 2 * LT;
 };
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 } // namespace bpx

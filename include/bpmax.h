@@ -240,6 +240,10 @@ typedef struct {
     int intra_wide;  /* in-tile closure launches of at most this many tiles use 1024-lane
                         workgroups (one cell per lane: latency), larger ones 256-lane
                         workgroups (four cells per lane: throughput)                    */
+    int outer_kgroup; /* K values (storage tiles of the inner sum) per R0 work item; 0:
+                        all of an output tile's.  1 keeps the items of one triangle in
+                        step, so each source tile is read from HBM once per k1 and then
+                        served from L2 to all its consumers (default 1)                  */
 } bpmax_tuning;
 /* Time every candidate setting on `npairs` synthetic M x N pairs (median of `reps` runs
  * after a warm-up each), keep the fastest for the class and return it. */
