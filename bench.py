@@ -84,12 +84,25 @@ def cpu_baseline(M, N, s1, s2, threads):
                                       f"{secs:.2f} s"}
 
 
-def pair_leg(B, eng, M, N, steps, warmup):
+def parse_schedule(text):
+    """'pipe:2:4096:0:64:2' -> tuning dict (schedule, outer_split, outer_items, graphs,
+    intra_wide, outer_kgroup)."""
+    f = text.split(":")
+    return {"schedule": f[0], "outer_split": int(f[1]), "outer_items": int(f[2]), "graphs": bool(int(f[3])),
+            "intra_wide": int(f[4]), "outer_kgroup": int(f[5])}
+
+
+def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
     s1, s2 = workload(M, N)
     cells = B.cell_updates(M, N)
     t0 = time.perf_counter()
-    tuned = eng.tune(M, N, 1, reps=2)  # launch schedule of this problem class (setup, untimed)
-    log(f"autotuned {M}x{N} in {time.perf_counter() - t0:.1f} s: {tuned}")
+    if schedule:  # a fixed launch schedule (profiling runs: the same launches every run)
+        tuned = parse_schedule(schedule)
+        eng.set_tuning(M, N, 1, tuned)
+        log(f"fixed schedule {tuned}")
+    else:
+        tuned = eng.tune(M, N, 1, reps=2)  # launch schedule of this problem class (setup, untimed)
+        log(f"autotuned {M}x{N} in {time.perf_counter() - t0:.1f} s: {tuned}")
     log(f"warmup {warmup} x {M}x{N}")
     score = None
     for _ in range(warmup):
@@ -316,6 +329,8 @@ def main():
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and isolated-R0 legs")
+    ap.add_argument("--schedule", default=None, metavar="S:SPLIT:ITEMS:GRAPHS:WIDE:KGROUP",
+                    help="fixed launch schedule of the pair leg instead of autotuning it, e.g. pipe:2:4096:0:64:2")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (reads MIB, writes MIB): the "
@@ -339,7 +354,7 @@ def main():
     import bpmaxcpu_amd as B
     M, N = args.M, args.N
     eng = B.Engine(0)
-    res = pair_leg(B, eng, M, N, args.steps, args.warmup)
+    res = pair_leg(B, eng, M, N, args.steps, args.warmup, args.schedule)
     batched = r0 = serving = None
     if not args.no_extra:
         batched = batched_leg(B, eng)
