@@ -100,6 +100,8 @@ def load_library(path: str = LIB_PATH):
         "bpmax_server_release": (c_int, [c_void_p, ctypes.c_longlong]),
         "bpmax_server_get_stats": (c_int, [c_void_p, ctypes.POINTER(ServerStats)]),
         "bpmax_server_stop": (c_int, [c_void_p]),
+        "bpmax_schedule_check": (c_int, [c_int, ip, ip, ctypes.POINTER(Tuning), c_int,
+                                         ctypes.POINTER(ctypes.c_longlong), ctypes.c_char_p, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -123,6 +125,21 @@ class Tuning(ctypes.Structure):
     def from_dict(tuning: dict) -> "Tuning":
         return Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
                       1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256), tuning.get("outer_kgroup", 1))
+
+
+def schedule_check(shapes, tuning: dict, skip: int = -1):
+    """Host-only check of the launch schedule pairs of these (M, N) shapes get under
+    `tuning` (bpmax_schedule_check): returns (rc, dependencies checked, message)."""
+    L = load_library()
+    Ms = np.array([m for m, _ in shapes], np.int32)
+    Ns = np.array([n for _, n in shapes], np.int32)
+    t = Tuning.from_dict(tuning)
+    nchk = ctypes.c_longlong(0)
+    msg = ctypes.create_string_buffer(256)
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.bpmax_schedule_check(len(shapes), Ms.ctypes.data_as(P), Ns.ctypes.data_as(P), ctypes.byref(t), skip,
+                                ctypes.byref(nchk), msg, len(msg))
+    return rc, nchk.value, msg.value.decode()
 
 
 class ServerConfig(ctypes.Structure):

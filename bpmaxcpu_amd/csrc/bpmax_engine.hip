@@ -17,7 +17,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "bpmax.h"
@@ -929,6 +931,174 @@ static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, 
         ctx->graph_key = gk;
     }
     HIPCHK(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
+    return BPMAX_OK;
+}
+
+// ------------------------------------------------------------------ schedule check
+// Host only.  Builds the launch list the pipelined (or one-stream) runner executes for the
+// given pair shapes and settings and checks, under the runner's ordering rules, that every
+// work item starts after all the work it reads has finished:
+//   R0 of F(i1,j1) storage tile (I,J): the closure of F(i1,k1) and F(k1+1,j1), k1 in
+//     [i1,j1), over the logical tiles of storage rows/columns I..J;
+//   closure of a logical tile: every R0 item of its storage tile, and the closure of the
+//     tiles left of it in its row and below it in its column; cross-tile before in-tile;
+//   R0 of the first outer diagonal: all of S2 (of its tile-size group).
+// The pipelined runner's rules, as enqueue_pipe applies them: stream order on the R0
+// stream and on the closure stream; an R0 range of d1 waits for the closure of d1-1 up to
+// logical diagonal rs*Ds+rs-1 of its last storage diagonal Ds (d1 = 0: for S2), and a
+// closure level DL of d1 waits for the R0 range of d1 holding storage diagonal
+// ceil(DL/rs).  skip >= 0 drops that launch's wait (a planted error for the check's test).
+extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, const bpmax_tuning *t, int skip,
+                                    long long *checked, char *msg, int msglen)
+{
+    auto fail = [&](const std::string &m) {
+        if (msg && msglen > 0) snprintf(msg, (size_t)msglen, "%s", m.c_str());
+        return BPMAX_EINVAL;
+    };
+    if (npairs < 1 || !Ms || !Ns || !tuning_valid(t)) return fail("bad arguments");
+    std::vector<PairDesc> P(npairs);
+    for (int p = 0; p < npairs; ++p) {
+        if (Ms[p] < 1 || Ns[p] < 1) return fail("bad shape");
+        PairDesc d{};
+        d.M = Ms[p];
+        d.N = Ns[p];
+        d.T = choose_tile(d.N);
+        d.nb = (d.N + d.T - 1) / d.T;
+        d.nbl = d.nb * (d.T / inner_tile(d.T));
+        P[p] = d;
+    }
+    const bool one = t->schedule == BPMAX_SCHED_LAUNCH;
+    Schedule S;
+    build_schedule(P, S, one ? 1 : t->outer_split, t->outer_items, t->outer_kgroup);
+    const int n = (int)S.launches.size();
+    // the launch each launch waits for (-1: none), as enqueue_pipe derives it
+    std::vector<int> wait(n, -1);
+    if (!one) {
+        std::map<std::tuple<int, int, int>, int> intra_of, outer_of; // (group, d1, diag) -> launch
+        std::vector<int> s2_end(S.groups.size(), -1);
+        for (int i = 0; i < n; ++i) {
+            const Launch &L = S.launches[i];
+            if (L.d1 < 0) s2_end[L.group] = i;
+            else if (L.kind == BPMAX_K_OUTER) outer_of[{L.group, L.d1, L.diag}] = i;
+            else if (L.kind == BPMAX_K_INNER_INTRA) intra_of[{L.group, L.d1, L.diag}] = i;
+        }
+        int cur_g = -1, cur_d1 = -1, in_wait = -1, out_wait = -1;
+        for (int i = 0; i < n; ++i) {
+            const Launch &L = S.launches[i];
+            if (L.d1 < 0) continue;
+            const GroupInfo &G = S.groups[L.group];
+            if (L.group != cur_g || L.d1 != cur_d1) {
+                cur_g = L.group;
+                cur_d1 = L.d1;
+                in_wait = out_wait = -1;
+            }
+            if (L.kind == BPMAX_K_OUTER) {
+                if (L.d1 == 0) {
+                    if (in_wait < 0) wait[i] = s2_end[L.group];
+                    in_wait = G.maxnbl;
+                } else {
+                    const int need = std::min(G.rs * L.diag + G.rs - 1, G.maxnbl - 1);
+                    if (need > in_wait) {
+                        wait[i] = intra_of.at({L.group, L.d1 - 1, need});
+                        in_wait = need;
+                    }
+                }
+            } else {
+                const int need = G.out_end[std::min((L.diag + G.rs - 1) / G.rs, G.maxnb - 1)];
+                if (need > out_wait) {
+                    wait[i] = outer_of.at({L.group, L.d1, need});
+                    out_wait = need;
+                }
+            }
+        }
+        if (skip >= 0 && skip < n) wait[skip] = -1;
+    }
+    // happens-before frontier: every R0 launch <= hbO[i] and every closure launch <=
+    // hbI[i] has finished when launch i starts (one stream: everything before i)
+    std::vector<int> hbO(n, -1), hbI(n, -1);
+    int prevO = -1, prevI = -1;
+    for (int i = 0; i < n; ++i) {
+        const bool outer = !one && S.launches[i].kind == BPMAX_K_OUTER;
+        int o = -1, in = -1;
+        auto absorb = [&](int j) {
+            if (j < 0) return;
+            o = std::max(o, hbO[j]);
+            in = std::max(in, hbI[j]);
+            if (!one && S.launches[j].kind == BPMAX_K_OUTER) o = std::max(o, j);
+            else in = std::max(in, j);
+        };
+        absorb(outer ? prevO : prevI);
+        if (wait[i] >= i) return fail("a launch waits for a later one");
+        absorb(wait[i]);
+        hbO[i] = o;
+        hbI[i] = in;
+        (outer ? prevO : prevI) = i;
+    }
+    auto before = [&](int a, int b) {
+        return (!one && S.launches[a].kind == BPMAX_K_OUTER) ? a <= hbO[b] : a <= hbI[b];
+    };
+    using Key = std::tuple<int, int, int, int, int>;
+    std::map<Key, int> closed, ext_at, r0_last;
+    std::vector<int> s2_last(S.groups.size(), -1);
+    for (int i = 0; i < n; ++i) {
+        const Launch &L = S.launches[i];
+        if (L.d1 < 0) s2_last[L.group] = std::max(s2_last[L.group], i);
+        for (int k = L.first; k < L.first + L.count; ++k) {
+            const WorkItem &w = S.items[k];
+            const Key key{w.pair, w.i1, w.j1, w.I, w.J};
+            if (L.kind == BPMAX_K_OUTER) r0_last[key] = std::max(r0_last.count(key) ? r0_last[key] : -1, i);
+            else if (L.d1 >= 0 && L.kind == BPMAX_K_INNER_INTRA) {
+                if (closed.count(key)) return fail("a tile is closed twice");
+                closed[key] = i;
+            } else if (L.d1 >= 0) ext_at[key] = std::max(ext_at.count(key) ? ext_at[key] : -1, i);
+        }
+    }
+    for (int p = 0; p < npairs; ++p)
+        for (int i1 = 0; i1 < P[p].M; ++i1)
+            for (int j1 = i1; j1 < P[p].M; ++j1)
+                for (int X = 0; X < P[p].nbl; ++X)
+                    for (int Y = X; Y < P[p].nbl; ++Y)
+                        if (!closed.count(Key{p, i1, j1, X, Y})) return fail("a tile is never closed");
+    long long nchk = 0;
+    for (int i = 0; i < n; ++i) {
+        const Launch &L = S.launches[i];
+        for (int k = L.first; k < L.first + L.count; ++k) {
+            const WorkItem &w = S.items[k];
+            const PairDesc &p = P[w.pair];
+            const int rs = p.nbl / p.nb;
+            char where[160];
+            snprintf(where, sizeof where, " (launch %d, pair %d, triangle (%d,%d), tile (%d,%d))", i, w.pair, w.i1, w.j1,
+                     w.I, w.J);
+            if (L.kind == BPMAX_K_OUTER) {
+                if (w.j1 == w.i1 && !before(s2_last[L.group], i)) return fail(std::string("R0 before S2") + where);
+                // closure goes by logical diagonal, and the closure order is checked below,
+                // so the last-closed tile of the square read suffices
+                for (int k1 = w.i1; k1 < w.j1; ++k1)
+                    for (int side = 0; side < 2; ++side) {
+                        const int a = side ? k1 + 1 : w.i1, b = side ? w.j1 : k1;
+                        const Key src{w.pair, a, b, w.I * rs, std::min((w.J + 1) * rs, p.nbl) - 1};
+                        if (!before(closed.at(src), i)) return fail(std::string("R0 before the closure it reads") + where);
+                        ++nchk;
+                    }
+            } else if (L.d1 >= 0) {
+                const auto r0 = r0_last.find(Key{w.pair, w.i1, w.j1, w.I / rs, w.J / rs});
+                if (r0 == r0_last.end() || !before(r0->second, i)) return fail(std::string("closure before R0") + where);
+                for (int Y = w.I; Y < w.J; ++Y)
+                    if (!before(closed.at(Key{w.pair, w.i1, w.j1, w.I, Y}), i))
+                        return fail(std::string("closure before the tile left of it") + where);
+                for (int X = w.I + 1; X <= w.J; ++X)
+                    if (!before(closed.at(Key{w.pair, w.i1, w.j1, X, w.J}), i))
+                        return fail(std::string("closure before the tile below it") + where);
+                if (L.kind == BPMAX_K_INNER_INTRA) {
+                    const auto e = ext_at.find(Key{w.pair, w.i1, w.j1, w.I, w.J});
+                    if (e != ext_at.end() && !before(e->second, i))
+                        return fail(std::string("in-tile before cross-tile") + where);
+                }
+                nchk += 1 + (w.J - w.I) * 2;
+            }
+        }
+    }
+    if (checked) *checked = nchk;
     return BPMAX_OK;
 }
 

@@ -265,6 +265,14 @@ int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);
 /* The hipStream_t the context launches on (as void*). */
 void *bpmax_ctx_stream(bpmax_ctx *ctx);
 int bpmax_ctx_synchronize(bpmax_ctx *ctx);
+/* Host-only check of the launch schedule that pairs of these shapes get under settings t:
+ * builds the launch list and verifies, under the runner's ordering rules (stream order plus
+ * the pipelined schedule's event waits), that every work item starts after all the work it
+ * reads.  skip >= 0 drops that launch's wait first (to test the check itself).
+ * *checked: dependencies verified.  BPMAX_EINVAL with a message in msg on a violation.
+ * No counterpart in the reference (its OpenMP schedule is implicit in its loops). */
+int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, const bpmax_tuning *t, int skip,
+                         long long *checked, char *msg, int msglen);
 /* Cell updates of the full recurrence for an MxN pair:
  *   C(M+1,3)C(N+1,3) + 2C(M+1,2)C(N+1,3) + 2C(M+1,3)C(N+1,2)  (R0 + R1/R2 + R3/R4). */
 double bpmax_cell_updates(int M, int N);
