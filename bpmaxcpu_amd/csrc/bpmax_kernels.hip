@@ -212,27 +212,30 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
 //       below / column left / corner, loaded up front)
 // (for S2 on a diagonal tile SL = SR = X itself, and S2 starts every upper cell at 0).
 //
-// Push formulation: each lane owns the 4 cells (q + 8m, b), m = 0..3, of column b =
-// tid % LT (q = tid / LT) and keeps their running maxima in registers.  Step s finalizes
+// Push formulation: each lane owns NC cells (q + m*QN, b), m < NC, of column b = tid % LT
+// (q = tid / LT) and keeps their running maxima in registers.  Step s finalizes
 // anti-diagonal s (owners write max(acc, P2) to LDS), one barrier, then every lane folds
 // in the terms whose newest factor just became final: for each owned cell the one new
 // cell of its row, X[a][a+s-(LT-1)] (R1), and the one new cell of its column,
 // X[b-s+(LT-1)][b] (R2), each with its S2 partner.  A partner that is not final yet still
 // holds a valid lower bound (every stored value is the score of some structure), so the
 // early sum never exceeds the optimum, and the exact pair is added when the later factor
-// finalizes.  Out-of-range factors read -inf pads (negative columns map to pad column
-// LT, rows past LT-1 are clamped to the pad row LT); indices past a cell's own
-// anti-diagonal only touch cells that are already final, whose registers are no longer
-// stored (row-end overruns stay inside the LDS layout).  All LDS
-// reads of a step are conflict-free: row factors are broadcasts, column factors and S2
-// partners run along consecutive columns.
+// finalizes.  Terms whose indices leave the tile must add nothing:
+//   * row factors of negative column: X rows have stride LX = 2*LT+4, so columns -LT..-1
+//     of row a are pad columns of row a-1 (or of a guard row), all -inf;
+//   * column factors of rows past LT-1: the row index is clamped to the -inf pad row LT;
+//   * every other out-of-tile index (partners of -inf factors; factors of cells already
+//     final, whose registers are no longer stored) only has to stay inside the LDS block,
+//     which the guard row before Sa and the order Sa, Sb, Xs ensure.
+// All LDS reads of a step are conflict-free: row factors are broadcasts, column factors
+// and S2 partners run along consecutive columns.
 template <int T, bool MODE_S, int NT = kThreads>
 __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
     constexpr int LT = Inner<T>::LT;
     using C = Intra<LT>;
-    constexpr int PAD = C::PAD, L = C::L;
+    constexpr int L = C::L, LX = C::LX;
     float *Xs = smem + C::XS, *Sa = smem + C::SA, *Sb = smem + C::SB;
     float *below = smem + C::BELOW, *left = below + LT, *cornerp = left + LT;
     float *wl = smem + C::WL;
@@ -269,17 +272,14 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         vcc = bc[J * LT + tid];
     }
     const float vw = tid < 25 ? w[tid] : 0.f;
-    // -inf pads: columns LT..L-1 of every row, row LT of Xs and Sb
-    for (int e = tid; e < (LT + PAD) * (L - LT); e += NT) {
-        const int a = e / (L - LT), b = LT + e % (L - LT);
-        Xs[a * L + b] = NEG_INF;
-        Sb[a * L + b] = NEG_INF;
-        if (a < LT) Sa[a * L + b] = NEG_INF;
-    }
-    for (int e = tid; e < PAD * LT; e += NT) {
-        const int a = LT + e / LT, b = e % LT;
-        Xs[a * L + b] = NEG_INF;
-        Sb[a * L + b] = NEG_INF;
+    // -inf: the guards, pad columns LT.. of every row of Sa, Sb and Xs, pad rows LT
+    // (everything before BELOW that is not a tile cell: no overlap with the tile stores)
+    for (int e = tid; e < C::FILL_END; e += NT) {
+        bool cell;
+        if (e >= C::XS) cell = (e - C::XS) % LX < LT && (e - C::XS) / LX < LT;
+        else if (e >= C::SB) cell = (e - C::SB) % L < LT && (e - C::SB) / L < LT;
+        else cell = e >= C::SA && (e - C::SA) % L < LT && (e - C::SA) / L < LT;
+        if (!cell) smem[e] = NEG_INF;
     }
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -293,8 +293,8 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
             if (r <= c + 2) x.z = fmaxf(x.z, 0.f);
             if (r <= c + 3) x.w = fmaxf(x.w, 0.f);
         }
-        *(float4 *)(Xs + a * L + c4) = x;
-        *(float4 *)(Sa + a * L + c4) = va[p];               // S2 tile (I,I)
+        *(float4 *)(Xs + a * LX + c4) = x;
+        *(float4 *)(Sa + a * L + c4) = va[p];                    // S2 tile (I,I)
         *(float4 *)(Sb + a * L + c4) = pick(diag, va[p], vb[p]); // S2 tile (J,J)
     }
     if (tid < LT) {
@@ -307,137 +307,99 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     if (tid < 25) wl[tid] = vw;
     // SL = S2 (I,I), SR = S2 (J,J) (both in Sb on a diagonal tile); S2's own diagonal tiles
     // take both factors from the tile being computed
+    const bool fresh = MODE_S && diag;
     const float *SL = diag ? (MODE_S ? Xs : Sb) : Sa;
-    const float *SR = diag && MODE_S ? Xs : Sb;
+    const float *SR = fresh ? Xs : Sb;
+    const int LSL = fresh ? LX : L, LSR = fresh ? LX : L; // their row strides
     __syncthreads();
 
     constexpr int NC = LT * LT / NT, QN = NT / LT; // cells per lane, lanes per column
     static_assert(NT % LT == 0 && NC * QN == LT, "cell mapping");
     const int b = tid % LT, q = tid / LT;
-    float acc[NC], wtv[NC];
-    bool p2ok[NC];
+    // per cell: running maximum, P2 weight (-inf where P2 does not apply, so that
+    // max(acc, p2 + w) needs no select), finalize step, P2 factor address
+    float acc[NC], wv[NC];
+    int sown[NC];
+    const float *p2p[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
         const int a = q + m * QN;
-        acc[m] = Xs[a * L + b];
-        wtv[m] = wl[5 * cr[a] + cc[b]]; // P2 weight intra(b[I*LT+a], b[J*LT+b])
-        p2ok[m] = (J * LT + b) - (I * LT + a) >= 4;
+        acc[m] = Xs[a * LX + b];
+        const bool ok = (J * LT + b) - (I * LT + a) >= 4;
+        wv[m] = ok ? wl[5 * cr[a] + cc[b]] : NEG_INF; // P2 weight intra(b[I*LT+a], b[J*LT+b])
+        sown[m] = b + (LT - 1) - a;
+        const int row = a + 1, col = b - 1; // P2 factor X[a+1][b-1], or across the tile edge
+        p2p[m] = row < LT ? (col >= 0 ? Xs + row * LX + col : left + row) : (col >= 0 ? below + col : cornerp);
     }
-    // the P2 factor of the owned cell on anti-diagonal s (its neighbour across the tile
-    // edge when the cell lies on the bottom row or left column)
-    auto p2_addr = [&](int s) -> const float * {
-        const int af = b + (LT - 1) - s;
-        if (af < 0 || af >= LT) return cornerp; // no cell of this column on s: unused
-        if (af + 1 < LT && b >= 1) return Xs + (af + 1) * L + b - 1;
-        if (b >= 1) return below + b - 1;
-        if (af + 1 < LT) return left + af + 1;
-        return cornerp;
-    };
-    // finalize the owned cell of anti-diagonal s, if any: max(acc, P2) to LDS
-    auto finalize = [&](int s, float p2) {
-        const int af = b + (LT - 1) - s;
-        if (af < q || af >= LT || (af - q) % QN != 0) return;
-        const int m = (af - q) / QN;
-        float v = NEG_INF, w2 = 0.f;
-        bool ok = false;
-#pragma unroll
-        for (int mm = 0; mm < NC; ++mm) {
-            v = mm == m ? acc[mm] : v;
-            w2 = mm == m ? wtv[mm] : w2;
-            ok = mm == m ? p2ok[mm] : ok;
-        }
-        if (ok) v = fmaxf(v, p2 + w2);
-        Xs[af * L + b] = v;
-    };
-    // S2 partners of the pushes of anti-diagonal s: constant tiles, read a step ahead --
-    // except for S2's own diagonal tiles, whose partners are cells of the tile itself and
-    // may finalize on the same anti-diagonal as the factor they pair with
-    const bool fresh = MODE_S && diag;
-    float sr[NC], sl[NC];
-    auto partners = [&](int s, float (&r)[NC], float (&l)[NC]) {
-        const int ac = b - s + (LT - 1);
-        const int ksl = ac - 1 < 0 ? 0 : ac - 1;
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            const int a = q + m * QN;
-            const int kr = a + s - (LT - 1);
-            const int ksr = clampi(kr + 1, 0, LT);
-            r[m] = SR[ksr * L + b];
-            l[m] = SL[a * L + ksl];
-        }
-    };
     const int s0 = diag ? LT - 1 : 0;
-    if constexpr (NC == 1) {
-        // one cell per lane: (a, b) finalizes on anti-diagonal sown, with its P2 factor at
-        // a fixed address; every step is reads, one fused push and a predicated store
-        const int a = q, sown = b + (LT - 1) - a;
-        const float *p2p = p2_addr(sown);
-        float v = acc[0];
-        const float wv = wtv[0];
-        const bool ok = p2ok[0];
-        float *own = Xs + a * L + b;
-        if (s0 == sown) *own = ok ? fmaxf(v, *p2p + wv) : v;
-        float srv = 0.f, slv = 0.f;
-        auto partner = [&](int st, float &r, float &l) {
-            const int kr = a + st - (LT - 1), ac = b - st + (LT - 1);
-            r = SR[clampi(kr + 1, 0, LT) * L + b];
-            l = SL[a * L + (ac - 1 < 0 ? 0 : ac - 1)];
-        };
-        if (!fresh) partner(s0, srv, slv);
-        // the P2 factor read at step s serves the finalize of step s+1 (it lies on
-        // anti-diagonal sown-2, final from step sown-2 on): never waited for on the spot
-        float p2 = *p2p;
+    // cells of anti-diagonal s0 start final
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+        if (s0 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], *p2p[m] + wv[m]);
+    // the P2 factors read at step s serve the finalize of step s+1 (the factor of a cell
+    // on anti-diagonal s+2 lies on anti-diagonal s)
+    float p2[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) p2[m] = *p2p[m];
+    if constexpr (!MODE_S) {
+        // F tiles: S2 partners at constant stride L.  Per step one pointer per factor kind
+        // moves by a constant; cells of one lane sit QN rows apart (immediate offsets).
+        const float *pxr = Xs + q * LX + q - (LT - 1) + s0;           // X[q][q+s-(LT-1)]
+        const float *psr = SR + (q + s0 - (LT - 2)) * L + b;          // SR[q+s-(LT-2)][b]
+        const float *psl = SL + q * L + b - s0 + (LT - 2);            // SL[q][b-s+(LT-2)]
         for (int s = s0; s < 2 * LT - 2; ++s) {
             __syncthreads(); // anti-diagonal s is final
-            const int kr = a + s - (LT - 1), ac = b - s + (LT - 1);
-            const float xr = Xs[a * L + (kr < 0 ? LT : kr)]; // negative: -inf pad column
-            const float xcol = Xs[clampi(ac, 0, LT) * L + b]; // past the last row: pad row LT
-            const float p2n = *p2p;
-            float srn = 0.f, sln = 0.f;
-            if (fresh) partner(s, srv, slv);
-            else partner(s + 1, srn, sln); // in flight across this step's arithmetic
-            v = max3(v, xr + srv, slv + xcol);
-            const float fin = fmaxf(v, ok ? p2 + wv : NEG_INF);
-            if (s + 1 == sown) *own = fin;
-            p2 = p2n;
-            if (!fresh) {
-                srv = srn;
-                slv = sln;
+            const float xcol = Xs[min(b - s + (LT - 1), LT) * LX + b]; // past the last row: pad row LT
+            float xr[NC], sr[NC], sl[NC], p2n[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                xr[m] = pxr[m * QN * (LX + 1)];
+                sr[m] = psr[m * QN * L];
+                sl[m] = psl[m * QN * L];
+                p2n[m] = *p2p[m];
             }
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
+                const float fin = fmaxf(acc[m], p2[m] + wv[m]);
+                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fin;
+                p2[m] = p2n[m];
+            }
+            pxr += 1;
+            psr += L;
+            psl -= 1;
         }
     } else {
-    finalize(s0, *p2_addr(s0));
-    partners(s0, sr, sl);
-    for (int s = s0; s < 2 * LT - 2; ++s) {
-        __syncthreads(); // anti-diagonal s is final
-        // the factors anti-diagonal s just completed: one per owned row, one per column
-        const int ac = b - s + (LT - 1);
-        const float xcol = Xs[clampi(ac, 0, LT) * L + b];
-        float xr[NC];
+        // S2 tiles: partners may be cells of the tile itself (fresh), read after the
+        // barrier and with clamped indices
+        for (int s = s0; s < 2 * LT - 2; ++s) {
+            __syncthreads(); // anti-diagonal s is final
+            const int ac = b - s + (LT - 1);
+            const float xcol = Xs[min(ac, LT) * LX + b];
+            const int ksl = ac - 1 < 0 ? 0 : ac - 1;
+            float xr[NC], sr[NC], sl[NC], p2n[NC];
 #pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            const int kr = q + m * QN + s - (LT - 1);
-            xr[m] = Xs[(q + m * QN) * L + (kr < 0 ? LT : kr)]; // negative: -inf pad column
+            for (int m = 0; m < NC; ++m) {
+                const int a = q + m * QN, kr = a + s - (LT - 1);
+                xr[m] = Xs[a * LX + kr];
+                sr[m] = SR[clampi(kr + 1, 0, LT) * LSR + b];
+                sl[m] = SL[a * LSL + ksl];
+                p2n[m] = *p2p[m];
+            }
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
+                const float fin = fmaxf(acc[m], p2[m] + wv[m]);
+                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fin;
+                p2[m] = p2n[m];
+            }
         }
-        const float p2n = *p2_addr(s + 1); // anti-diagonal s-1: final
-        float srn[NC], sln[NC];
-        if (fresh) partners(s, sr, sl);
-        partners(s + 1, srn, sln);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) acc[m] = max3(acc[m], xr[m] + sr[m], sl[m] + xcol);
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            sr[m] = srn[m];
-            sl[m] = sln[m];
-        }
-        finalize(s + 1, p2n);
-    }
     }
     __syncthreads();
     float *Xo = ltile<T, LT>(X, I, J, nb);
     for (int idx = tid; idx < CH; idx += NT) {
         const int a = idx / Q, c4 = (idx % Q) * 4;
-        *(float4 *)(Xo + a * T + c4) = *(const float4 *)(Xs + a * L + c4);
+        *(float4 *)(Xo + a * T + c4) = *(const float4 *)(Xs + a * LX + c4);
     }
 }
 

@@ -274,17 +274,27 @@ __device__ __forceinline__ float lane_group_max(float v)
     return v;
 }
 
-// LDS layout of the in-tile closure on an LT x LT tile: one extra -inf row (row LT) after
-// Xs and Sb and L - LT = 4 -inf columns per row (see intra_body), then the boundary
-// rows/columns, the weight table and the codes.  About 15 KB at LT = 32, so that a
-// 256-lane closure workgroup fits on a CU beside four R0 workgroups.
+// LDS layout of the in-tile closure on an LT x LT tile, in this order: a guard of L
+// floats, Sa (LT rows of stride L = LT+4: the S2 tile (I,I)), Sb (LT+1 rows: S2 (J,J) and a
+// pad row), a guard row of LX = 2*LT+4 floats, Xs (LT+1 rows of stride LX: the tile and a
+// pad row; columns LT.. of every row are pads), then the boundary row/column/corner, the
+// weight table and the codes.  Everything before BELOW starts at -inf (pads and guards)
+// and the tiles are stored over it.  About 19 KB at LT = 32, so that a 256-lane closure
+// workgroup fits on a CU beside four R0 workgroups.
 template <int LT>
 struct Intra {
-    static constexpr int PAD = 1, L = LT + 4;
-    static constexpr int XS = 0, SA = (LT + PAD) * L, SB = SA + LT * L;
-    static constexpr int BELOW = SB + (LT + PAD) * L, WL = BELOW + 2 * LT + 4, CODES = WL + 28;
+    static constexpr int L = LT + 4, LX = 2 * LT + 4;
+    static constexpr int SA = L, SB = SA + LT * L, XS = SB + (LT + 1) * L + LX;
+    static constexpr int BELOW = XS + (LT + 1) * LX, FILL_END = BELOW;
+    static constexpr int WL = BELOW + 2 * LT + 4, CODES = WL + 28;
     static constexpr int FLOATS = CODES + 2 * LT;
 };
+
+// acc = max(acc, x, y) in place, one v_max3_f32.
+__device__ __forceinline__ void max3_into(float &acc, float x, float y)
+{
+    asm("v_max3_f32 %0, %0, %1, %2" : "+v"(acc) : "v"(x), "v"(y));
+}
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
