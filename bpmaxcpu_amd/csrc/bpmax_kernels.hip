@@ -336,11 +336,16 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
 #pragma unroll
     for (int m = 0; m < NC; ++m)
         if (s0 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], *p2p[m] + wv[m]);
-    // the P2 factors read at step s serve the finalize of step s+1 (the factor of a cell
-    // on anti-diagonal s+2 lies on anti-diagonal s)
-    float p2[NC];
+    // the P2 factor read at step s serves the finalize of step s+1: it belongs to the
+    // lane's cell on anti-diagonal s+2 (if any) and lies on anti-diagonal s
+    auto p2_next = [&](int s) -> const float * {
+        const int d = sown[0] - (s + 2); // that cell is m = d / QN
+        const float *p = cornerp;
 #pragma unroll
-    for (int m = 0; m < NC; ++m) p2[m] = *p2p[m];
+        for (int m = 0; m < NC; ++m) p = d == m * QN ? p2p[m] : p;
+        return p;
+    };
+    float p2 = *p2_next(s0 - 1);
     if constexpr (!MODE_S) {
         // F tiles: S2 partners at constant stride L.  Per step one pointer per factor kind
         // moves by a constant; cells of one lane sit QN rows apart (immediate offsets).
@@ -350,21 +355,20 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         for (int s = s0; s < 2 * LT - 2; ++s) {
             __syncthreads(); // anti-diagonal s is final
             const float xcol = Xs[min(b - s + (LT - 1), LT) * LX + b]; // past the last row: pad row LT
-            float xr[NC], sr[NC], sl[NC], p2n[NC];
+            float xr[NC], sr[NC], sl[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
                 xr[m] = pxr[m * QN * (LX + 1)];
                 sr[m] = psr[m * QN * L];
                 sl[m] = psl[m * QN * L];
-                p2n[m] = *p2p[m];
             }
+            const float p2n = *p2_next(s);
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
                 max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
-                const float fin = fmaxf(acc[m], p2[m] + wv[m]);
-                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fin;
-                p2[m] = p2n[m];
+                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], p2 + wv[m]);
             }
+            p2 = p2n;
             pxr += 1;
             psr += L;
             psl -= 1;
@@ -377,22 +381,21 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
             const int ac = b - s + (LT - 1);
             const float xcol = Xs[min(ac, LT) * LX + b];
             const int ksl = ac - 1 < 0 ? 0 : ac - 1;
-            float xr[NC], sr[NC], sl[NC], p2n[NC];
+            float xr[NC], sr[NC], sl[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
                 const int a = q + m * QN, kr = a + s - (LT - 1);
                 xr[m] = Xs[a * LX + kr];
                 sr[m] = SR[clampi(kr + 1, 0, LT) * LSR + b];
                 sl[m] = SL[a * LSL + ksl];
-                p2n[m] = *p2p[m];
             }
+            const float p2n = *p2_next(s);
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
                 max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
-                const float fin = fmaxf(acc[m], p2[m] + wv[m]);
-                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fin;
-                p2[m] = p2n[m];
+                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], p2 + wv[m]);
             }
+            p2 = p2n;
         }
     }
     __syncthreads();
