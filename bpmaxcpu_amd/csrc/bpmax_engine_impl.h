@@ -92,7 +92,7 @@ struct bpmax_ctx {
     int outer_items = 2048;              // work items an R0 launch aims for (split-K granularity)
     int outer_kgroup = 1;                // K values per R0 work item (0: every K of the tile)
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
-                                         // 1024-lane (latency) form, larger ones the 256-lane
+                                         // 1024-lane (latency) form, larger ones the 512-lane
     int eff_intra_wide = 256;            // the current run's threshold
     std::map<long long, bpx::Tuning> tuning; // tuned settings per problem class
     std::vector<hipEvent_t> sync_ev;

@@ -444,9 +444,10 @@ __global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__
 
 // The closure kernels (cross-tile and in-tile) are the critical path of the pipelined
 // schedule and run beside its R0 launches.  They fit on a CU next to four R0 workgroups
-// (at most 64 VGPRs, i.e. 8 waves per SIMD, and 9 / 15 KB of LDS; the 256-lane in-tile
-// form for F) and raise their wave priority, so they start at once and issue ahead of
-// the R0 waves instead of queueing behind them.
+// (at most 64 VGPRs per SIMD lane between them, i.e. 8 waves of up to 64 or 2 of up to
+// 32 each, and 9 / 19 KB of LDS; the 512-lane in-tile form for F) and raise their wave
+// priority, so they start at once and issue ahead of the R0 waves instead of queueing
+// behind them.
 template <int T, bool MODE_S>
 __global__ __launch_bounds__(kThreads, 8) void k_inner_ext(const PairDesc *__restrict__ pairs,
                                                         const WorkItem *__restrict__ items, float *__restrict__ arena)
@@ -459,8 +460,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_inner_ext(const PairDesc *__res
 
 // The in-tile closure in two widths: NT = 1024 lanes (one cell per lane, 4 waves per SIMD
 // to interleave the short dependent chain of each anti-diagonal step: lowest latency, for
-// launches of few tiles) or NT = 256 (four cells per lane, the per-step index work shared:
-// highest throughput, for launches of many tiles).
+// launches of few tiles) or NT = 512 (two cells per lane: the per-step pointer and barrier
+// work shared, half the waves per tile, for launches of many tiles).  S2 uses 1024 or 256.
 template <int T, bool MODE_S, int NT>
 __global__ __launch_bounds__(NT, MODE_S ? 1 : 8) void k_inner_intra(const PairDesc *__restrict__ pairs,
                                                     const WorkItem *__restrict__ items,
@@ -850,7 +851,7 @@ hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const P
     if (single_strand) {
         if (wide) { BPX_INTRA(true, 1024); } else { BPX_INTRA(true, kThreads); }
     } else {
-        if (wide) { BPX_INTRA(false, 1024); } else { BPX_INTRA(false, kThreads); }
+        if (wide) { BPX_INTRA(false, 1024); } else { BPX_INTRA(false, 2 * kThreads); }
     }
 #undef BPX_INTRA
     return hipGetLastError();

@@ -279,7 +279,7 @@ __device__ __forceinline__ float lane_group_max(float v)
 // pad row), a guard row of LX = 2*LT+4 floats, Xs (LT+1 rows of stride LX: the tile and a
 // pad row; columns LT.. of every row are pads), then the boundary row/column/corner, the
 // weight table and the codes.  Everything before BELOW starts at -inf (pads and guards)
-// and the tiles are stored over it.  About 19 KB at LT = 32, so that a 256-lane closure
+// and the tiles are stored over it.  About 19 KB at LT = 32, so that a 512-lane closure
 // workgroup fits on a CU beside four R0 workgroups.
 template <int LT>
 struct Intra {

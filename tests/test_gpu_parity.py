@@ -34,7 +34,7 @@ def eng():
 def eng2(request):
     """An engine under each schedule: two-stream pipeline (graph-replayed and eager; R0 work
     items of one K each), per-phase launches on one stream with the in-tile closure always
-    256-lane (narrow; R0 items over every K of a tile) or always 1024-lane (wide; R0 items of
+    512-lane (narrow; R0 items over every K of a tile) or always 1024-lane (wide; R0 items of
     two K each, the last one ragged), and the dependency-tile scheduler."""
     e = B.Engine(0)
     sched = request.param.split("-")[0]
