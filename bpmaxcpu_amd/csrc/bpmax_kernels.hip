@@ -76,31 +76,35 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
     FetchA<T> fa;
     FetchB<T> fb;
     float4 brow = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF); // row I*T of F(k1+1,j1), lanes < Q
+    // S1 terms of the fetched step (R3: S1(i1,k1), R4: S1(k1+1,j1)), loaded with its tiles so
+    // that the step's wait for them is the wait for the tiles, not one for the next prefetch
+    float s1a_f = NEG_INF, s1b_f = NEG_INF;
     auto fetch = [&](int s) {
         const int k1 = i1 + s / nK, K = Kb + s % nK;
         const float *Bt = tri_at(arena, P, k1 + 1, j1);
         fetch_A<T, T>(fa, tri_at(arena, P, i1, k1), nb, I, K, 0);
         fetch_B<T, T>(fb, Bt, nb, K, J, 1);
         if (K == I && threadIdx.x < Q) brow = *(const float4 *)(Bt + toff + 4 * threadIdx.x);
+        if (K == I) s1a_f = S1[i1 * M + k1];
+        if (K == J) s1b_f = S1[(k1 + 1) * M + j1];
     };
     if (it.step_begin < it.step_end) fetch(it.step_begin);
     for (int s = it.step_begin; s < it.step_end; ++s) {
-        const int k1 = i1 + s / nK, K = Kb + s % nK;
+        const int K = Kb + s % nK;
         __syncthreads();
         commit_A<T>(As, fa, 0, false);
         commit_B<T>(Bs, fb);
         if (K == I && threadIdx.x < Q) *(float4 *)(Bs - LD + 4 * threadIdx.x) = brow;
+        const float s1a = s1a_f, s1b = s1b_f;
         __syncthreads();
         if (s + 1 < it.step_end) fetch(s + 1); // in flight while this step computes
         if (K == J) { // R4: As[c][r] = F(i1,k1)[I*T + r][J*T + c]
-            const float s1b = S1[(k1 + 1) * M + j1];
 #pragma unroll
             for (int i = 0; i < R; ++i)
 #pragma unroll
                 for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], As[(c0 + j) * LD + r0 + i] + s1b);
         }
         if (K == I) { // R3: Bs[r-1][c] = F(k1+1,j1)[I*T + r][J*T + c]
-            const float s1a = S1[i1 * M + k1];
 #pragma unroll
             for (int i = 0; i < R; ++i)
 #pragma unroll
