@@ -85,11 +85,11 @@ def cpu_baseline(M, N, s1, s2, threads):
 
 
 def parse_schedule(text):
-    """'pipe:2:4096:0:64:2' -> tuning dict (schedule, outer_split, outer_items, graphs,
-    intra_wide, outer_kgroup)."""
+    """'pipe:2:4096:0:64:2[:1]' -> tuning dict (schedule, outer_split, outer_items, graphs,
+    intra_wide, outer_kgroup[, closure_ahead])."""
     f = text.split(":")
     return {"schedule": f[0], "outer_split": int(f[1]), "outer_items": int(f[2]), "graphs": bool(int(f[3])),
-            "intra_wide": int(f[4]), "outer_kgroup": int(f[5])}
+            "intra_wide": int(f[4]), "outer_kgroup": int(f[5]), "closure_ahead": len(f) > 6 and f[6] == "1"}
 
 
 def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
@@ -154,7 +154,8 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
     # one-stream schedule, for the phase breakdown without overlap
     eng.set_tuning(M, N, 1, {"schedule": "launch", "outer_split": 1, "outer_items": tuned["outer_items"],
                              "graphs": False, "intra_wide": tuned["intra_wide"],
-                             "outer_kgroup": tuned.get("outer_kgroup", 1)})
+                             "outer_kgroup": tuned.get("outer_kgroup", 1),
+                             "closure_ahead": tuned.get("closure_ahead", False)})
     eng.set_schedule(B.SCHED_LAUNCH)
     eng.solve(s1, s2, layout=B.LAYOUT_NONE)
     eng.set_timing(True)
@@ -329,7 +330,7 @@ def main():
     ap.add_argument("--N", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and isolated-R0 legs")
-    ap.add_argument("--schedule", default=None, metavar="S:SPLIT:ITEMS:GRAPHS:WIDE:KGROUP",
+    ap.add_argument("--schedule", default=None, metavar="S:SPLIT:ITEMS:GRAPHS:WIDE:KGROUP[:AHEAD]",
                     help="fixed launch schedule of the pair leg instead of autotuning it, e.g. pipe:2:4096:0:64:2")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

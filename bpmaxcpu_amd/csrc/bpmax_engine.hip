@@ -149,6 +149,7 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
     }
     if (const char *e = getenv("BPMAX_OUTER_SPLIT")) c->outer_split = std::max(0, atoi(e));
     if (const char *e = getenv("BPMAX_OUTER_KGROUP")) c->outer_kgroup = std::max(0, atoi(e));
+    if (const char *e = getenv("BPMAX_CLOSURE_AHEAD")) c->closure_ahead = atoi(e) != 0;
     *out = c;
     return BPMAX_OK;
 }
@@ -240,7 +241,8 @@ long long tuning_class(int M, int N, int npairs)
 static bool tuning_valid(const bpmax_tuning *t)
 {
     return t && (t->schedule == BPMAX_SCHED_PIPE || t->schedule == BPMAX_SCHED_LAUNCH) && t->outer_split >= 0 &&
-           t->outer_items >= 64 && t->intra_wide >= 0 && t->outer_kgroup >= 0 && t->outer_kgroup <= 255;
+           t->outer_items >= 64 && t->intra_wide >= 0 && t->outer_kgroup >= 0 && t->outer_kgroup <= 255 &&
+           (t->closure_ahead == 0 || t->closure_ahead == 1);
 }
 } // namespace bpx
 
@@ -253,8 +255,8 @@ extern "C" int bpmax_ctx_set_tuning(bpmax_ctx *ctx, int M, int N, int npairs, co
         return BPMAX_OK;
     }
     if (!tuning_valid(t)) return BPMAX_EINVAL;
-    ctx->tuning[cls] =
-        Tuning{t->schedule, t->outer_split, t->outer_items, t->graphs ? 1 : 0, t->intra_wide, t->outer_kgroup};
+    ctx->tuning[cls] = Tuning{t->schedule,   t->outer_split,  t->outer_items,  t->graphs ? 1 : 0,
+                              t->intra_wide, t->outer_kgroup, t->closure_ahead};
     return BPMAX_OK;
 }
 
@@ -267,6 +269,7 @@ extern "C" int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *
     ctx->graphs = t->graphs != 0;
     ctx->intra_wide = t->intra_wide;
     ctx->outer_kgroup = t->outer_kgroup;
+    ctx->closure_ahead = t->closure_ahead;
     return BPMAX_OK;
 }
 
@@ -275,8 +278,8 @@ extern "C" int bpmax_ctx_get_tuning(bpmax_ctx *ctx, int M, int N, int npairs, bp
     if (!ctx || !t || M < 1 || N < 1 || npairs < 1) return BPMAX_EINVAL;
     auto it = ctx->tuning.find(tuning_class(M, N, npairs));
     if (it == ctx->tuning.end()) return BPMAX_ENOTFOUND;
-    *t = bpmax_tuning{it->second.schedule, it->second.outer_split, it->second.outer_items, it->second.graphs,
-                      it->second.intra_wide, it->second.outer_kgroup};
+    *t = bpmax_tuning{it->second.schedule,   it->second.outer_split,  it->second.outer_items,  it->second.graphs,
+                      it->second.intra_wide, it->second.outer_kgroup, it->second.closure_ahead};
     return BPMAX_OK;
 }
 
@@ -301,18 +304,18 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
     };
     // two rounds: the launch schedule at the default in-tile width threshold, then the
     // threshold (largest in-tile launch that uses the 1024-lane form) under the best schedule
-    const int w0 = ctx->intra_wide, k0 = ctx->outer_kgroup;
+    const int w0 = ctx->intra_wide, k0 = ctx->outer_kgroup, a0 = ctx->closure_ahead;
     std::vector<Tuning> cands;
     if (npairs == 1) {
         for (int items : {1024, 2048, 4096}) {
-            cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, items, 0, w0, k0});
-            for (int split : {1, 2, 4, 0}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0});
+            cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, items, 0, w0, k0, a0});
+            for (int split : {1, 2, 4, 0}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0, a0});
         }
-        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 2, 2048, 1, w0, k0});
+        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 2, 2048, 1, w0, k0, a0});
     } else {
-        cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, 2048, 0, w0, k0});
+        cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, 2048, 0, w0, k0, a0});
         for (int items : {2048, 4096})
-            for (int split : {1, 2, 4}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0});
+            for (int split : {1, 2, 4}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0, a0});
     }
     const long long cls = tuning_class(mM, mN, npairs);
     const bool had = ctx->tuning.count(cls) != 0;
@@ -354,6 +357,13 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
         c.outer_kgroup = kg;
         rc = trial(c);
     }
+    const Tuning kg_best = best;
+    for (int ahead : {0, 1}) {
+        if (rc != BPMAX_OK || ahead == a0) continue;
+        Tuning c = kg_best;
+        c.closure_ahead = ahead;
+        rc = trial(c);
+    }
     if (rc != BPMAX_OK) {
         if (had) ctx->tuning[cls] = saved;
         else ctx->tuning.erase(cls);
@@ -361,8 +371,8 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
     }
     ctx->tuning[cls] = best;
     if (best_out)
-        *best_out = bpmax_tuning{best.schedule, best.outer_split, best.outer_items, best.graphs, best.intra_wide,
-                                 best.outer_kgroup};
+        *best_out = bpmax_tuning{best.schedule,   best.outer_split,  best.outer_items,  best.graphs,
+                                 best.intra_wide, best.outer_kgroup, best.closure_ahead};
     if (best_ms) *best_ms = best_t;
     return BPMAX_OK;
 }
@@ -480,6 +490,18 @@ static void chunked(std::vector<WorkItem> &v, WorkItem base, int steps, int ch, 
 }
 
 
+// Items of `ch` steps over [sb, se) of one tile (cross-tile steps in the closure launches
+// that carry them ahead), all max-combining.
+static void chunked_range(std::vector<WorkItem> &v, WorkItem base, int sb, int se, int ch)
+{
+    for (int s = sb; s < se; s += ch) {
+        WorkItem w = base;
+        w.step_begin = s;
+        w.step_end = std::min(se, s + ch);
+        v.push_back(w);
+    }
+}
+
 static int pick_chunk(long long total_steps, long long tiles)
 {
     // aim for ~8 work items per CU (256 CUs) when the work allows it
@@ -496,11 +518,16 @@ static int pick_chunk(long long total_steps, long long tiles)
 //   for every outer diagonal d1: R0 launches over ranges of storage block diagonals (by
 //   default one launch per storage diagonal D: all storage tiles (I,I+D) of every triangle
 //   on d1; outer_split > 0 merges them into that many ranges), then the inner closure by
-//   logical block diagonal DL (cross-tile launch, closure launch).
+//   logical block diagonal DL (cross-tile launch, closure launch) -- or, with
+//   closure_ahead, one launch per DL (k_inner_closure): the in-tile items of DL, whose
+//   workgroups first run their tiles' cross-tile steps D-1 and D (the two that read
+//   diagonal DL-1), and cross-tile items with the other steps of the tiles on DL+1 (they
+//   read diagonals below DL only).
 // Every range / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
 // runner can attach its events to each.  An R0 launch's `diag` is the last storage
 // diagonal of its range.
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup)
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
+                    bool closure_ahead)
 {
     S.items.clear();
     S.launches.clear();
@@ -602,7 +629,7 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
                 emit(BPMAX_K_OUTER, g, false, f, upd, d1, Db);
             }
             for (int D = 0; D < G.maxnbl; ++D) {
-                if (D > 0) {
+                if (D > 0 && !closure_ahead) {
                     size_t f = S.items.size();
                     long long tot = 0;
                     for (size_t p = 0; p < P.size(); ++p)
@@ -618,13 +645,36 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
                     emit(BPMAX_K_INNER_EXT, g, false, f, 0, d1, D);
                 }
                 size_t f = S.items.size();
+                const int tin = closure_ahead ? TASK_INTRA << WI_TYPE_SHIFT : 0;
                 for (size_t p = 0; p < P.size(); ++p) {
                     if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D) continue;
                     for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
                         for (int I = 0; I + D < P[p].nbl; ++I)
-                            S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, 0});
+                            S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, tin});
+                }
+                if (closure_ahead) { // steps [0, D) and [D+2, 2D+2) of the tiles on D+1
+                    const int D1 = D + 1;
+                    long long tot = 0;
+                    for (size_t p = 0; p < P.size(); ++p)
+                        if (P[p].T == T && P[p].M > d1 && P[p].nbl > D1)
+                            tot += (long long)(P[p].M - d1) * (P[p].nbl - D1) * 2 * D;
+                    // longer items than the cross-tile launches': a 512-lane item's fixed cost
+                    // (staging latency, workgroup start) is large next to its lean steps
+                    // (64x512: 8 steps 70.8 ms, 2 steps 86.5 ms; 24x512: 8 steps 11.8 ms, 5 steps 14.2 ms)
+                    const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
+                    const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
+                    for (size_t p = 0; p < P.size() && D > 0; ++p) {
+                        if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D1) continue;
+                        for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                            for (int I = 0; I + D1 < P[p].nbl; ++I) {
+                                const WorkItem base{(int)p, i1, i1 + d1, I, I + D1, 0, 0, tex};
+                                chunked_range(S.items, base, 0, D, ch);
+                                chunked_range(S.items, base, D + 2, 2 * D1, ch);
+                            }
+                    }
                 }
                 emit(BPMAX_K_INNER_INTRA, g, false, f, 0, d1, D);
+                S.launches.back().ahead = closure_ahead;
             }
         }
     }
@@ -783,6 +833,9 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
         return timed(ctx, st, L.kind, L.updates,
                      [&] { return launch_inner_ext(st, L.T, L.single_strand, dP, items, L.count, arena); });
     default:
+        if (L.ahead)
+            return timed(ctx, st, L.kind, L.updates,
+                         [&] { return launch_inner_closure(st, L.T, dP, items, L.count, arena, dcodes, dw); });
         return timed(ctx, st, L.kind, L.updates, [&] {
             return launch_inner_intra(st, L.T, L.single_strand, dP, items, L.count, arena, dcodes, dw,
                                       L.count <= ctx->eff_intra_wide);
@@ -811,6 +864,14 @@ static int ensure_sync_events(bpmax_ctx *ctx, const Schedule &S, std::vector<siz
         ctx->sync_ev.push_back(e);
     }
     return BPMAX_OK;
+}
+
+// Last storage diagonal whose R0 a closure launch needs: the tiles it writes lie on logical
+// diagonal L.diag (and L.diag+1 when it carries cross-tile steps ahead); logical diagonal
+// DL lies in storage diagonals floor(DL/rs) .. ceil(DL/rs).
+static int closure_r0_diag(const GroupInfo &G, const Launch &L)
+{
+    return std::min((L.diag + (L.ahead ? 1 : 0) + G.rs - 1) / G.rs, G.maxnb - 1);
 }
 
 // Pipelined schedule on two streams: sO runs the fill and every R0 launch, sI runs S1, S2
@@ -876,7 +937,7 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
             if ((rc = run_launch(ctx, sO, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
             HIPCHK(ctx, hipEventRecord(ev_out(L.group, L.d1, L.diag), sO));
         } else {
-            const int need = G.out_end[std::min((L.diag + G.rs - 1) / G.rs, G.maxnb - 1)];
+            const int need = G.out_end[closure_r0_diag(G, L)];
             if (need > out_wait) {
                 HIPCHK(ctx, hipStreamWaitEvent(sI, ev_out(L.group, L.d1, need), 0));
                 out_wait = need;
@@ -940,8 +1001,13 @@ static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, 
 // work item starts after all the work it reads has finished:
 //   R0 of F(i1,j1) storage tile (I,J): the closure of F(i1,k1) and F(k1+1,j1), k1 in
 //     [i1,j1), over the logical tiles of storage rows/columns I..J;
-//   closure of a logical tile: every R0 item of its storage tile, and the closure of the
-//     tiles left of it in its row and below it in its column; cross-tile before in-tile;
+//   in-tile closure of a logical tile: every R0 item of its storage tile, and the closure
+//     of the tiles left of it in its row and below it in its column; cross-tile before
+//     in-tile;
+//   cross-tile step s of tile (I,J), D = J-I: R0 of its storage tile and the closure of the
+//     one tile it reads, (I,I+s) for s < D, else (I+1+s-D,J);
+//   every cross-tile step of every tile run once (by a cross-tile item, or, with
+//     closure_ahead, steps D-1 and D by the tile's in-tile item);
 //   R0 of the first outer diagonal: all of S2 (of its tile-size group).
 // The pipelined runner's rules, as enqueue_pipe applies them: stream order on the R0
 // stream and on the closure stream; an R0 range of d1 waits for the closure of d1-1 up to
@@ -969,7 +1035,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
     }
     const bool one = t->schedule == BPMAX_SCHED_LAUNCH;
     Schedule S;
-    build_schedule(P, S, one ? 1 : t->outer_split, t->outer_items, t->outer_kgroup);
+    build_schedule(P, S, one ? 1 : t->outer_split, t->outer_items, t->outer_kgroup, t->closure_ahead != 0);
     const int n = (int)S.launches.size();
     // the launch each launch waits for (-1: none), as enqueue_pipe derives it
     std::vector<int> wait(n, -1);
@@ -1004,7 +1070,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                     }
                 }
             } else {
-                const int need = G.out_end[std::min((L.diag + G.rs - 1) / G.rs, G.maxnb - 1)];
+                const int need = G.out_end[closure_r0_diag(G, L)];
                 if (need > out_wait) {
                     wait[i] = outer_of.at({L.group, L.d1, need});
                     out_wait = need;
@@ -1047,7 +1113,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
             const WorkItem &w = S.items[k];
             const Key key{w.pair, w.i1, w.j1, w.I, w.J};
             if (L.kind == BPMAX_K_OUTER) r0_last[key] = std::max(r0_last.count(key) ? r0_last[key] : -1, i);
-            else if (L.d1 >= 0 && L.kind == BPMAX_K_INNER_INTRA) {
+            else if (L.d1 >= 0 && L.kind == BPMAX_K_INNER_INTRA && wi_type(w.flags) != TASK_EXT) {
                 if (closed.count(key)) return fail("a tile is closed twice");
                 closed[key] = i;
             } else if (L.d1 >= 0) ext_at[key] = std::max(ext_at.count(key) ? ext_at[key] : -1, i);
@@ -1059,6 +1125,39 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                 for (int X = 0; X < P[p].nbl; ++X)
                     for (int Y = X; Y < P[p].nbl; ++Y)
                         if (!closed.count(Key{p, i1, j1, X, Y})) return fail("a tile is never closed");
+    // cross-tile step coverage of F tiles: each of the 2D steps exactly once
+    std::map<Key, std::vector<char>> steps_run;
+    auto run_step = [&](const Key &k, int D, int st) -> bool {
+        auto &v = steps_run[k];
+        if (v.empty()) v.assign(2 * D, 0);
+        if (st < 0 || st >= 2 * D || v[st]) return false;
+        v[st] = 1;
+        return true;
+    };
+    for (int i = 0; i < n; ++i) {
+        const Launch &L = S.launches[i];
+        if (L.d1 < 0 || L.kind == BPMAX_K_OUTER) continue;
+        for (int k = L.first; k < L.first + L.count; ++k) {
+            const WorkItem &w = S.items[k];
+            const Key key{w.pair, w.i1, w.j1, w.I, w.J};
+            const int D = w.J - w.I;
+            bool ok = true;
+            if (L.kind == BPMAX_K_INNER_EXT || wi_type(w.flags) == TASK_EXT)
+                for (int st = w.step_begin; st < w.step_end; ++st) ok = ok && run_step(key, D, st);
+            else if (L.ahead && D > 0)
+                ok = run_step(key, D, D - 1) && run_step(key, D, D);
+            if (!ok) return fail("a cross-tile step is run twice or out of range");
+        }
+    }
+    for (int p = 0; p < npairs; ++p)
+        for (int i1 = 0; i1 < P[p].M; ++i1)
+            for (int j1 = i1; j1 < P[p].M; ++j1)
+                for (int X = 0; X < P[p].nbl; ++X)
+                    for (int Y = X + 1; Y < P[p].nbl; ++Y) {
+                        const auto it = steps_run.find(Key{p, i1, j1, X, Y});
+                        if (it == steps_run.end() || std::count(it->second.begin(), it->second.end(), 1) != 2 * (Y - X))
+                            return fail("a cross-tile step is never run");
+                    }
     long long nchk = 0;
     for (int i = 0; i < n; ++i) {
         const Launch &L = S.launches[i];
@@ -1080,6 +1179,18 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                         if (!before(closed.at(src), i)) return fail(std::string("R0 before the closure it reads") + where);
                         ++nchk;
                     }
+            } else if (L.d1 >= 0 && (L.kind == BPMAX_K_INNER_EXT || wi_type(w.flags) == TASK_EXT)) {
+                // cross-tile steps: step s reads X tile (I, I+s) (R1, s < D) or
+                // (I+1+s-D, J) (R2); it writes the tile, after its R0
+                const auto r0 = r0_last.find(Key{w.pair, w.i1, w.j1, w.I / rs, w.J / rs});
+                if (r0 == r0_last.end() || !before(r0->second, i)) return fail(std::string("closure before R0") + where);
+                const int D = w.J - w.I;
+                for (int st = w.step_begin; st < w.step_end; ++st) {
+                    const Key src = st < D ? Key{w.pair, w.i1, w.j1, w.I, w.I + st}
+                                           : Key{w.pair, w.i1, w.j1, w.I + 1 + st - D, w.J};
+                    if (!before(closed.at(src), i)) return fail(std::string("cross-tile step before its operand") + where);
+                    ++nchk;
+                }
             } else if (L.d1 >= 0) {
                 const auto r0 = r0_last.find(Key{w.pair, w.i1, w.j1, w.I / rs, w.J / rs});
                 if (r0 == r0_last.end() || !before(r0->second, i)) return fail(std::string("closure before R0") + where);
@@ -1189,8 +1300,8 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     std::sort(Ts.begin(), Ts.end());
 
     // Launch-schedule settings: the tuned ones of this problem class, else the context's.
-    Tuning eff{ctx->schedule, ctx->outer_split, ctx->outer_items, ctx->graphs ? 1 : 0, ctx->intra_wide,
-               ctx->outer_kgroup};
+    Tuning eff{ctx->schedule,   ctx->outer_split,  ctx->outer_items,  ctx->graphs ? 1 : 0,
+               ctx->intra_wide, ctx->outer_kgroup, ctx->closure_ahead};
     if (ctx->schedule != BPMAX_SCHED_TASKS) {
         int mM = 0, mN = 0;
         for (auto &d : P) {
@@ -1204,7 +1315,8 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
     ctx->eff_intra_wide = eff.intra_wide;
 
     // The schedule is rebuilt only when the shapes or the schedule settings change.
-    std::vector<long long> key = {sched, ctx->task_chunk, eff.outer_split, eff.outer_items, eff.outer_kgroup, npairs};
+    std::vector<long long> key = {sched,           ctx->task_chunk,   eff.outer_split, eff.outer_items,
+                                  eff.outer_kgroup, eff.closure_ahead, npairs};
     // (intra_wide changes kernel choices, not the schedule: the graph key carries it)
     for (auto &d : P) {
         key.push_back(d.M);
@@ -1229,7 +1341,7 @@ int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int
         } else {
             // one stream: a single R0 launch per outer diagonal (nothing to overlap with)
             build_schedule(P, S, sched == BPMAX_SCHED_LAUNCH ? 1 : eff.outer_split, eff.outer_items,
-                           eff.outer_kgroup);
+                           eff.outer_kgroup, eff.closure_ahead != 0);
         }
         ctx->sched_key = key;
     }

@@ -25,11 +25,13 @@ struct Launch {
     int group;           // T group (pairs sharing a storage tile size)
     int d1;              // outer diagonal (-1: S2 phases)
     int diag;            // storage block diagonal (outer) or logical block diagonal (inner)
+    bool ahead = false;  // in-tile launch that also carries diagonal diag+1's early
+                         // cross-tile steps (k_inner_closure)
 };
 
 // launch-schedule settings of one problem class (bpmax_tuning)
 struct Tuning {
-    int schedule, outer_split, outer_items, graphs, intra_wide, outer_kgroup;
+    int schedule, outer_split, outer_items, graphs, intra_wide, outer_kgroup, closure_ahead;
 };
 
 struct GroupInfo {
@@ -61,7 +63,8 @@ struct Server; // continuous-batching server (bpmax_server.cpp)
 int choose_tile(int N);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
-void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup);
+void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
+                    bool closure_ahead);
 long long tuning_class(int M, int N, int npairs);
 unsigned long long splitmix64_host(unsigned long long x);
 void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
@@ -94,6 +97,7 @@ struct bpmax_ctx {
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
                                          // 1024-lane (latency) form, larger ones the 512-lane
     int eff_intra_wide = 256;            // the current run's threshold
+    int closure_ahead = 0;               // inner closure in one launch per diagonal (bpmax_tuning)
     std::map<long long, bpx::Tuning> tuning; // tuned settings per problem class
     std::vector<hipEvent_t> sync_ev;
     bool graphs = false;

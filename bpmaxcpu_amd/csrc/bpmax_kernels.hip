@@ -144,6 +144,25 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
 //   S: s <  D : S[I][K] (x) S[K*LT+1 ..][J], K = I+s; for K = I only column LT-1 of S[I][I]
 //               (the remaining columns of tile I pair with cells of (I,J) itself and are
 //               handled by the in-tile wavefront).
+// Step s of F reads X tile (I,I+s) (R1) or (I+1+s-D, J) (R2): logical diagonal s or
+// 2D-1-s, so only steps D-1 and D read diagonal D-1.
+struct ExtStep {
+    int aK, cshift;         // A: logical tile (I, aK) of X (a_x) or S, column shift cshift
+    int bK, rshift;         // B: rows of logical tile row bK shifted by rshift, column tile J
+    bool a_x, b_x;          // operand taken from X (else from S2)
+    bool lastcol, adiag, bdiag;
+};
+
+template <bool MODE_S>
+__device__ __forceinline__ ExtStep ext_step(int I, int J, int s)
+{
+    const int D = J - I;
+    if (MODE_S) return ExtStep{I + s, 0, I + s, 1, false, false, s == 0, false, false};
+    if (s < D) return ExtStep{I + s, 0, I + s, 1, true, false, false, s == 0, false}; // R1
+    const int K2 = I + 1 + (s - D);                                                    // R2
+    return ExtStep{K2, -1, K2, 0, false, true, false, false, K2 == J};
+}
+
 // Always max-combined into the tile (other items of the same tile run concurrently).
 template <int T, bool MODE_S>
 __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
@@ -153,7 +172,7 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
     float *As = smem, *Bs = smem + LT * LD;
     int r0, c0;
     lane_block<LT>(r0, c0);
-    const int I = it.I, J = it.J, nb = P.nb, D = J - I;
+    const int I = it.I, J = it.J, nb = P.nb;
     float *S = arena + P.S2;
     float *X = MODE_S ? S : tri_at(arena, P, it.i1, it.j1);
     float acc[R][R];
@@ -163,47 +182,105 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
         for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
     FetchA<LT> fa;
     FetchB<LT> fb;
-    // step s: its operand shifts and which operand is a diagonal tile
-    int cshift = 0;
-    bool lastcol = false, adiag = false, bdiag = false;
+    ExtStep st{};
     auto fetch = [&](int s) {
-        if (MODE_S) {
-            const int K = I + s;
-            cshift = 0;
-            lastcol = K == I;
-            adiag = bdiag = false;
-            fetch_A<T, LT>(fa, S, nb, I, K, 0);
-            fetch_B<T, LT>(fb, S, nb, K, J, 1);
-        } else if (s < D) { // R1: X(I,K) x S2(K,J) shifted down one row
-            const int K = I + s;
-            cshift = 0;
-            lastcol = false;
-            adiag = K == I;
-            bdiag = false;
-            fetch_A<T, LT>(fa, X, nb, I, K, 0);
-            fetch_B<T, LT>(fb, S, nb, K, J, 1);
-        } else {            // R2: S2(I,K2) shifted right one column x X(K2,J)
-            const int K2 = I + 1 + (s - D);
-            cshift = -1;
-            lastcol = false;
-            adiag = false;
-            bdiag = K2 == J;
-            fetch_A<T, LT>(fa, S, nb, I, K2, -1);
-            fetch_B<T, LT>(fb, X, nb, K2, J, 0);
-        }
+        st = ext_step<MODE_S>(I, J, s);
+        fetch_A<T, LT>(fa, st.a_x ? X : S, nb, I, st.aK, st.cshift);
+        fetch_B<T, LT>(fb, st.b_x ? X : S, nb, st.bK, J, st.rshift);
     };
     if (it.step_begin < it.step_end) fetch(it.step_begin);
     for (int s = it.step_begin; s < it.step_end; ++s) {
         __syncthreads();
-        commit_A<LT>(As, fa, cshift, lastcol);
+        commit_A<LT>(As, fa, st.cshift, st.lastcol);
         commit_B<LT>(Bs, fb);
         int kb, ke;
-        quad_k_range<LT>(adiag, bdiag, lastcol, kb, ke);
+        quad_k_range<LT>(st.adiag, st.bdiag, st.lastcol, kb, ke);
         __syncthreads();
         if (s + 1 < it.step_end) fetch(s + 1);
         product<LT>(acc, As, Bs, r0, c0, kb, ke);
     }
     store_acc<T, LT>(ltile<T, LT>(X, I, J, nb), acc, r0, c0, true);
+}
+
+// Steps [sb, se) of the cross-tile part of logical tile (I,J) of X = F(i1,j1) on 512 lanes
+// (two 256-lane halves, one staging buffer pair), lean enough for the 32 VGPRs a 512-lane
+// closure workgroup may use beside four R0 workgroups:
+//   * the lower half stages each step's A operand, row-major (As[r][k], k shifted by
+//     -cshift), the upper half its B operand (Bs[k][c]): one float4 per lane;
+//   * the halves take the lower and the upper half of the step's k range (quad_k_range
+//     gives multiples of QS = 16), one k per loop round (few live registers).
+// acc: this lane's 2 x 2 block maxima over its half of k.
+template <int T>
+__device__ void ext_acc512(const PairDesc &P, float *X, int I, int J, int sb, int se,
+                           float (&acc)[Tc<Inner<T>::LT>::R][Tc<Inner<T>::LT>::R], float *arena, float *smem)
+{
+    constexpr int LT = Inner<T>::LT;
+    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R, Q = Stage<LT>::Q;
+    static_assert(Stage<LT>::PER == 1 && R == 2 && LD % 4 == 0, "one staged float4 per lane, 2 x 2 blocks");
+    float *As = smem, *Bs = smem + LT * LD;
+    float *S = arena + P.S2;
+    const int nbl = P.nb * (T / LT);
+    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x / kThreads), tid = threadIdx.x % kThreads;
+    int r0, c0;
+    lane_block<LT>(r0, c0, tid);
+    const int sr = tid / Q, sc4 = (tid % Q) * 4; // staged row and first column (A: row r, B: row k)
+    float4 v = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
+    float prev = NEG_INF; // A with column shift -1: X[I*LT + r][aK*LT - 1], lanes r < LT
+    ExtStep st{};
+    auto fetch = [&](int s) {
+        st = ext_step<false>(I, J, s);
+        // tile bases are uniform; only B's last row can cross into the next tile
+        if (half == 0) {
+            const float *A = st.a_x ? X : S;
+            const bool have = st.aK >= I && st.aK < nbl;
+            const float4 x = *(const float4 *)(ltile<T, LT>(A, I, have ? st.aK : I, P.nb) + sr * T + sc4);
+            v = have ? x : make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
+            prev = NEG_INF;
+            if (st.cshift < 0 && st.aK - 1 >= I && tid < LT)
+                prev = ltile<T, LT>(A, I, st.aK - 1, P.nb)[tid * T + LT - 1];
+        } else {
+            const float *B = st.b_x ? X : S;
+            const bool ok1 = st.bK + 1 <= J && st.bK + 1 < nbl;
+            const float *t0 = ltile<T, LT>(B, st.bK, J, P.nb), *t1 = ltile<T, LT>(B, ok1 ? st.bK + 1 : J, J, P.nb);
+            const int row = sr + st.rshift;
+            const bool cross = row >= LT;
+            const float4 y = *(const float4 *)((cross ? t1 : t0 + row * T) + sc4);
+            v = cross && !ok1 ? make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF) : y;
+        }
+    };
+    if (sb < se) fetch(sb);
+    for (int s = sb; s < se; ++s) {
+        __syncthreads();
+        if (half == 0) {
+            float *row = As + sr * LD + sc4;
+            if (st.cshift == 0) {
+                *(float4 *)row = v;
+            } else { // As[r][k] = A[r][k-1]: columns shift right by one, column 0 from prev
+                row[1] = v.x;
+                row[2] = v.y;
+                row[3] = v.z;
+                if (sc4 + 4 < LT) row[4] = v.w;
+                if (tid < LT) As[tid * LD] = prev;
+            }
+        } else {
+            *(float4 *)(Bs + sr * LD + sc4) = v;
+        }
+        int kb, ke;
+        quad_k_range<LT>(st.adiag, st.bdiag, false, kb, ke, tid);
+        const int mid = kb + ((ke - kb) >> 1);
+        const int k0 = half ? mid : kb, k1 = half ? ke : mid;
+        __syncthreads();
+        if (s + 1 < se) fetch(s + 1);
+#pragma unroll 1
+        for (int k = k0; k < k1; ++k) {
+            const float a0 = As[r0 * LD + k], a1 = As[(r0 + 1) * LD + k];
+            const float2 b = *(const float2 *)(Bs + k * LD + c0);
+            acc[0][0] = fmaxf(acc[0][0], a0 + b.x);
+            acc[0][1] = fmaxf(acc[0][1], a0 + b.y);
+            acc[1][0] = fmaxf(acc[1][0], a1 + b.x);
+            acc[1][1] = fmaxf(acc[1][1], a1 + b.y);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------- inner intra
@@ -233,9 +310,14 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
 //     which the guard row before Sa and the order Sa, Sb, Xs ensure.
 // All LDS reads of a step are conflict-free: row factors are broadcasts, column factors
 // and S2 partners run along consecutive columns.
+//
+// late (F, 512 lanes): the tile's two cross-tile steps that read logical diagonal J-I-1
+// (R1 with K = J-1, R2 with K' = I+1) are run here first, when the launch leaves them out of
+// the cross-tile items (see k_inner_closure).
 template <int T, bool MODE_S, int NT = kThreads>
 __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
-                           const int *__restrict__ codes, const float *__restrict__ w, float *smem)
+                           const int *__restrict__ codes, const float *__restrict__ w, float *smem,
+                           bool late = false)
 {
     constexpr int LT = Inner<T>::LT;
     using C = Intra<LT>;
@@ -249,6 +331,23 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     float *S = arena + P.S2;
     float *X = MODE_S ? S : tri_at(arena, P, i1, j1);
     const int tid = threadIdx.x;
+
+    // late cross-tile steps: staged in the LDS that the set-up below overwrites, results
+    // kept in registers until the tile is in LDS
+    constexpr bool kLate = !MODE_S && NT == 2 * kThreads;
+    constexpr int RX = Tc<LT>::R;
+    float eacc[RX][RX];
+    const bool late_steps = kLate && late && J > I;
+    if constexpr (kLate) {
+        if (late_steps) {
+#pragma unroll
+            for (int i = 0; i < RX; ++i)
+#pragma unroll
+                for (int j = 0; j < RX; ++j) eacc[i][j] = NEG_INF;
+            ext_acc512<T>(P, X, I, J, J - I - 1, J - I + 1, eacc, arena, smem);
+            __syncthreads();
+        }
+    }
 
     // tiles: every global load issued before any LDS store (float4, unrolled, branch-free;
     // lanes past the tile's LT*Q float4 chunks re-read chunk 0 and store nothing)
@@ -309,6 +408,23 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     }
     if (tid == 0) cornerp[0] = cn;
     if (tid < 25) wl[tid] = vw;
+    if constexpr (kLate) {
+        if (late_steps) { // fold the late steps into the tile, one 256-lane half at a time
+            int er0, ec0;
+            lane_block<LT>(er0, ec0, tid % kThreads);
+            for (int h = 0; h < 2; ++h) {
+                __syncthreads();
+                if (tid / kThreads == h)
+#pragma unroll
+                    for (int i = 0; i < RX; ++i)
+#pragma unroll
+                        for (int j = 0; j < RX; ++j) {
+                            float *c = Xs + (er0 + i) * LX + ec0 + j;
+                            *c = fmaxf(*c, eacc[i][j]);
+                        }
+            }
+        }
+    }
     // SL = S2 (I,I), SR = S2 (J,J) (both in Sb on a diagonal tile); S2's own diagonal tiles
     // take both factors from the tile being computed
     const bool fresh = MODE_S && diag;
@@ -477,6 +593,48 @@ __global__ __launch_bounds__(NT, MODE_S ? 1 : 8) void k_inner_intra(const PairDe
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     intra_body<T, MODE_S, NT>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
 }
+
+// The inner closure of F by logical block diagonal DL in one launch per diagonal instead of
+// two (cross-tile, then in-tile): the launch for DL holds
+//   * in-tile items (type TASK_INTRA) of the tiles on DL, each of which first runs the two
+//     cross-tile steps of its tile that read diagonal DL-1 (intra_body, late), and
+//   * cross-tile items (type TASK_EXT) with the other steps of the tiles on DL+1, which
+//     read diagonals below DL only and so run beside the in-tile work of DL.
+// 512 lanes per workgroup, at most 32 VGPRs each (the bound below asks for that), so that
+// like k_inner_intra it fits beside four R0 workgroups.  (The bound's occupancy target of
+// 16 waves per SIMD is not met -- SGPRs limit it -- and need not be: only the VGPR budget
+// matters for running beside R0.)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
+template <int T>
+__global__ __launch_bounds__(2 * kThreads, 16) void k_inner_closure(const PairDesc *__restrict__ pairs,
+                                                                const WorkItem *__restrict__ items,
+                                                                float *__restrict__ arena,
+                                                                const int *__restrict__ codes,
+                                                                const float *__restrict__ w)
+{
+    constexpr int LT = Inner<T>::LT, R = Tc<LT>::R;
+    static_assert(Tc<LT>::PROD_FLOATS <= Intra<LT>::FLOATS, "staging fits the in-tile LDS");
+    __shared__ __attribute__((aligned(16))) float smem[Intra<LT>::FLOATS];
+    __builtin_amdgcn_s_setprio(3);
+    const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
+    const PairDesc &P = pairs[it.pair];
+    if (wi_type(it.flags) == TASK_EXT) {
+        float *X = tri_at(arena, P, it.i1, it.j1);
+        float acc[R][R];
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
+        ext_acc512<T>(P, X, it.I, it.J, it.step_begin, it.step_end, acc, arena, smem);
+        int r0, c0;
+        lane_block<LT>(r0, c0, threadIdx.x % kThreads);
+        store_acc<T, LT>(ltile<T, LT>(X, it.I, it.J, P.nb), acc, r0, c0, true);
+    } else {
+        intra_body<T, false, 2 * kThreads>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem, true);
+    }
+}
+#pragma clang diagnostic pop
 
 __global__ __launch_bounds__(kThreads) void k_s1(const PairDesc *__restrict__ pairs, float *__restrict__ arena,
                                                  const int *__restrict__ codes, const float *__restrict__ w)
@@ -858,6 +1016,15 @@ hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const P
         if (wide) { BPX_INTRA(false, 1024); } else { BPX_INTRA(false, 2 * kThreads); }
     }
 #undef BPX_INTRA
+    return hipGetLastError();
+}
+
+hipError_t launch_inner_closure(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n,
+                                float *arena, const int *codes, const float *w)
+{
+    if (n <= 0) return hipSuccess;
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_inner_closure<TT>, dim3(n), dim3(2 * kThreads), 0, st, pairs, items, arena,
+                                         codes, w));
     return hipGetLastError();
 }
 

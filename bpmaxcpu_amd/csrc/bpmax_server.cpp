@@ -331,6 +331,7 @@ extern "C" int bpmax_server_start(bpmax_ctx *ctx, const bpmax_server_config *cfg
         sl.ctx->outer_split = ctx->outer_split;
         sl.ctx->outer_items = ctx->outer_items;
         sl.ctx->outer_kgroup = ctx->outer_kgroup;
+        sl.ctx->closure_ahead = ctx->closure_ahead;
         sl.ctx->intra_wide = ctx->intra_wide;
         sl.ctx->graphs = false; // batches change shape every time
         sl.ctx->tuning = ctx->tuning; // tuned launch schedules of the owning context

@@ -40,12 +40,15 @@ struct Tc {
     static constexpr int PROD_FLOATS = 2 * LT * LD;
 };
 
+// The staging and product helpers below run on 256 lanes; `tid` is the lane's index among
+// them (threadIdx.x, or its index within a 256-lane half of a wider workgroup).
+
 // First row r0 and column c0 of this lane's R x R block.
 template <int LT>
-__device__ __forceinline__ void lane_block(int &r0, int &c0)
+__device__ __forceinline__ void lane_block(int &r0, int &c0, int tid = threadIdx.x)
 {
     constexpr int R = Tc<LT>::R, QS = Tc<LT>::QS, QL = Tc<LT>::QL;
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int w = tid / 64, l = tid % 64;
     r0 = (w >> 1) * QS + (l / QL) * R;
     c0 = (w & 1) * QS + (l % QL) * R;
 }
@@ -56,10 +59,11 @@ __device__ __forceinline__ void lane_block(int &r0, int &c0)
 //   lastcol: only column LT-1 of A is finite.
 // Multiples of QS; wave-uniform.
 template <int LT>
-__device__ __forceinline__ void quad_k_range(bool adiag, bool bdiag, bool lastcol, int &kb, int &ke)
+__device__ __forceinline__ void quad_k_range(bool adiag, bool bdiag, bool lastcol, int &kb, int &ke,
+                                             int tid = threadIdx.x)
 {
     constexpr int QS = Tc<LT>::QS;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
     kb = lastcol ? LT - QS : (adiag ? (w >> 1) * QS : 0);
     ke = bdiag ? ((w & 1) + 1) * QS : LT;
 }
@@ -109,16 +113,17 @@ struct FetchA {
 };
 
 template <int LT>
-__device__ __forceinline__ void a_coords(int p, int &r, int &c4)
+__device__ __forceinline__ void a_coords(int p, int &r, int &c4, int tid)
 {
-    const int idx = threadIdx.x + p * kThreads;
+    const int idx = tid + p * kThreads;
     const int B = idx / 64, l = idx % 64;
     r = 16 * (B % (LT / 16)) + l % 16;
     c4 = 4 * (4 * (B / (LT / 16)) + l / 16);
 }
 
 template <int T, int LT>
-__device__ __forceinline__ void fetch_A(FetchA<LT> &f, const float *X, int nb, int I, int K, int cshift)
+__device__ __forceinline__ void fetch_A(FetchA<LT> &f, const float *X, int nb, int I, int K, int cshift,
+                                        int tid = threadIdx.x)
 {
     const int nbl = nb * (T / LT);
     f.have = K >= I && K < nbl;
@@ -126,25 +131,26 @@ __device__ __forceinline__ void fetch_A(FetchA<LT> &f, const float *X, int nb, i
 #pragma unroll
     for (int p = 0; p < Stage<LT>::PER; ++p) {
         int r, c4;
-        a_coords<LT>(p, r, c4);
+        a_coords<LT>(p, r, c4, tid);
         f.v[p] = *(const float4 *)(t + r * T + c4);
     }
     f.have_prev = cshift < 0 && K - 1 >= I;
     f.prev = NEG_INF;
     if (cshift < 0) {
         const float *tp = ltile<T, LT>(X, I, f.have_prev ? K - 1 : I, nb);
-        f.prev = tp[(threadIdx.x % LT) * T + LT - 1];
+        f.prev = tp[(tid % LT) * T + LT - 1];
     }
 }
 
 template <int LT>
-__device__ __forceinline__ void commit_A(float *As, const FetchA<LT> &f, int cshift, bool lastcol_only)
+__device__ __forceinline__ void commit_A(float *As, const FetchA<LT> &f, int cshift, bool lastcol_only,
+                                         int tid = threadIdx.x)
 {
     constexpr int LD = Tc<LT>::LD;
 #pragma unroll
     for (int p = 0; p < Stage<LT>::PER; ++p) {
         int r, c4;
-        a_coords<LT>(p, r, c4);
+        a_coords<LT>(p, r, c4, tid);
         const float e[4] = {f.v[p].x, f.v[p].y, f.v[p].z, f.v[p].w};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -154,7 +160,7 @@ __device__ __forceinline__ void commit_A(float *As, const FetchA<LT> &f, int csh
             if (k < LT) As[k * LD + r] = val;
         }
     }
-    if (cshift < 0 && threadIdx.x < LT) As[threadIdx.x] = (f.have_prev && !lastcol_only) ? f.prev : NEG_INF;
+    if (cshift < 0 && tid < LT) As[tid] = (f.have_prev && !lastcol_only) ? f.prev : NEG_INF;
 }
 
 __device__ __forceinline__ float sel(bool ok, float v) { return ok ? v : NEG_INF; }
@@ -172,14 +178,15 @@ struct FetchB {
 };
 
 template <int T, int LT>
-__device__ __forceinline__ void fetch_B(FetchB<LT> &f, const float *X, int nb, int K, int J, int rshift)
+__device__ __forceinline__ void fetch_B(FetchB<LT> &f, const float *X, int nb, int K, int J, int rshift,
+                                        int tid = threadIdx.x)
 {
     constexpr int Q = Stage<LT>::Q;
     const int nbl = nb * (T / LT);
     f.have = 0;
 #pragma unroll
     for (int p = 0; p < Stage<LT>::PER; ++p) {
-        const int idx = threadIdx.x + p * kThreads;
+        const int idx = tid + p * kThreads;
         const int k = idx / Q, c4 = (idx % Q) * 4;
         const int x = K * LT + k + rshift;
         const int Xt = x / LT;
@@ -191,12 +198,12 @@ __device__ __forceinline__ void fetch_B(FetchB<LT> &f, const float *X, int nb, i
 }
 
 template <int LT>
-__device__ __forceinline__ void commit_B(float *Bs, const FetchB<LT> &f)
+__device__ __forceinline__ void commit_B(float *Bs, const FetchB<LT> &f, int tid = threadIdx.x)
 {
     constexpr int LD = Tc<LT>::LD, Q = Stage<LT>::Q;
 #pragma unroll
     for (int p = 0; p < Stage<LT>::PER; ++p) {
-        const int idx = threadIdx.x + p * kThreads;
+        const int idx = tid + p * kThreads;
         const int k = idx / Q, c4 = (idx % Q) * 4;
         // select per component: a select of whole float4 values is lowered through scratch
         const bool ok = (f.have >> p) & 1u;
@@ -215,15 +222,15 @@ __device__ __forceinline__ float max3(float a, float b, float c)
 }
 
 // acc[i][j] = max(acc[i][j], max_{kb<=k<ke} As[k][r0+i] + Bs[k][c0+j]) (kb, ke multiples
-// of QS): two k per round, v_add_f32 x2 + v_max3_f32 per pair of cell updates.
-template <int LT>
+// of KU, by default QS): two k per round, v_add_f32 x2 + v_max3_f32 per pair of cell updates.
+template <int LT, int KU = Tc<LT>::QS>
 __device__ __forceinline__ void product(float (&acc)[Tc<LT>::R][Tc<LT>::R], const float *As, const float *Bs,
                                         int r0, int c0, int kb, int ke)
 {
-    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R, QS = Tc<LT>::QS;
-    for (int k0 = kb; k0 < ke; k0 += QS) {
+    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R;
+    for (int k0 = kb; k0 < ke; k0 += KU) {
 #pragma unroll
-        for (int k = k0; k < k0 + QS; k += 2) {
+        for (int k = k0; k < k0 + KU; k += 2) {
             float av0[R], av1[R], bv0[R], bv1[R];
 #pragma unroll
             for (int i = 0; i < R; ++i) {

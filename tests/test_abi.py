@@ -134,23 +134,29 @@ def ctypes_ref(x):
     {"schedule": "pipe", "outer_split": 0, "outer_items": 4096, "outer_kgroup": 2},
     {"schedule": "pipe", "outer_split": 4, "outer_items": 64, "outer_kgroup": 3},
     {"schedule": "launch", "outer_split": 1, "outer_items": 2048, "outer_kgroup": 1},
+    {"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "outer_kgroup": 1, "closure_ahead": True},
+    {"schedule": "pipe", "outer_split": 0, "outer_items": 64, "outer_kgroup": 0, "closure_ahead": True},
+    {"schedule": "launch", "outer_split": 1, "outer_items": 2048, "outer_kgroup": 2, "closure_ahead": True},
 ])
 def test_schedule_runs_every_item_after_what_it_reads(tuning):
     """The launch list of a 40x300 pair and of a mixed batch (two tile sizes, ragged M and N)
-    is checked on the host: every R0 item after the closures it reads, every closure after
-    its R0 tile, the tiles left of / below it and its cross-tile part, R0 of the first
-    diagonal after S2 -- under the pipelined runner's stream order and event waits."""
+    is checked on the host: every R0 item after the closures it reads, every in-tile closure
+    after its R0 tile, the tiles left of / below it and its cross-tile part, every cross-tile
+    step after the tile it reads and each of them run once (also when the in-tile launches
+    carry them, closure_ahead), R0 of the first diagonal after S2 -- under the pipelined
+    runner's stream order and event waits."""
     for shapes in ([(40, 300)], [(12, 200), (7, 70), (20, 260), (3, 40), (1, 9), (5, 64)]):
         rc, n, msg = B.schedule_check(shapes, dict(tuning, graphs=False, intra_wide=256))
         assert rc == B.OK, msg
         assert n > 1000
 
 
-def test_schedule_check_catches_a_missing_wait():
+@pytest.mark.parametrize("ahead", [False, True])
+def test_schedule_check_catches_a_missing_wait(ahead):
     """Planted error: with the wait of one launch dropped, a dependency is no longer ordered
     and the check must say so -- for every launch whose wait carries one."""
     tuning = {"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "outer_kgroup": 1, "graphs": False,
-              "intra_wide": 256}
+              "intra_wide": 256, "closure_ahead": ahead}
     caught = 0
     for k in range(0, 700):
         rc, _, msg = B.schedule_check([(10, 200)], tuning, skip=k)

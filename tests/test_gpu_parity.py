@@ -30,20 +30,28 @@ def eng():
     e.close()
 
 
-@pytest.fixture(scope="module", params=["pipe", "pipe-eager", "launch-narrow", "launch-wide", "tasks"])
+@pytest.fixture(scope="module",
+                params=["pipe", "pipe-eager", "pipe-ahead", "launch-narrow", "launch-wide", "launch-ahead", "tasks"])
 def eng2(request):
     """An engine under each schedule: two-stream pipeline (graph-replayed and eager; R0 work
     items of one K each), per-phase launches on one stream with the in-tile closure always
     512-lane (narrow; R0 items over every K of a tile) or always 1024-lane (wide; R0 items of
-    two K each, the last one ragged), and the dependency-tile scheduler."""
+    two K each, the last one ragged), each of the two with the inner closure in one launch per
+    diagonal (ahead: cross-tile steps carried by the previous diagonal's launch and by the
+    in-tile workgroups; graph-replayed in the pipeline), and the dependency-tile scheduler."""
     e = B.Engine(0)
     sched = request.param.split("-")[0]
+    ahead = request.param.endswith("ahead")
     if sched == "launch":
         narrow = request.param.endswith("narrow")
         e.set_default_tuning({"schedule": "launch", "outer_split": 1, "outer_items": 2048, "graphs": False,
-                              "intra_wide": 0 if narrow else 1 << 30, "outer_kgroup": 0 if narrow else 2})
+                              "intra_wide": 0 if narrow else 1 << 30, "outer_kgroup": 0 if narrow else 2,
+                              "closure_ahead": ahead})
+    elif ahead:
+        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": True,
+                              "intra_wide": 256, "outer_kgroup": 1, "closure_ahead": True})
     e.set_schedule(sched)
-    e.set_graphs(request.param == "pipe")
+    e.set_graphs(request.param in ("pipe", "pipe-ahead"))
     yield e
     e.close()
 
@@ -355,9 +363,9 @@ def test_autotuned_schedule_is_used_and_exact():
             Fo, so = O.bpmax(s1, s2, intra, inter)
             assert score == so and _bits_equal(F, Fo)
         for setting in [{"schedule": "launch", "outer_split": 1, "outer_items": 64, "graphs": False,
-                         "intra_wide": 0, "outer_kgroup": 0},
+                         "intra_wide": 0, "outer_kgroup": 0, "closure_ahead": True},
                         {"schedule": "pipe", "outer_split": 0, "outer_items": 4096, "graphs": True,
-                         "intra_wide": 1 << 30, "outer_kgroup": 3}]:
+                         "intra_wide": 1 << 30, "outer_kgroup": 3, "closure_ahead": False}]:
             eng.set_tuning(6, 70, 1, setting)
             assert eng.get_tuning(6, 70, 1) == setting
             F, score = eng.solve(s1, s2, intra, inter)
