@@ -150,6 +150,7 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
     if (const char *e = getenv("BPMAX_OUTER_SPLIT")) c->outer_split = std::max(0, atoi(e));
     if (const char *e = getenv("BPMAX_OUTER_KGROUP")) c->outer_kgroup = std::max(0, atoi(e));
     if (const char *e = getenv("BPMAX_CLOSURE_AHEAD")) c->closure_ahead = atoi(e) != 0;
+    if (const char *e = getenv("BPMAX_INTRA_BLOCKED")) c->intra_blocked = atoi(e) != 0;
     *out = c;
     return BPMAX_OK;
 }
@@ -838,7 +839,7 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
                          [&] { return launch_inner_closure(st, L.T, dP, items, L.count, arena, dcodes, dw); });
         return timed(ctx, st, L.kind, L.updates, [&] {
             return launch_inner_intra(st, L.T, L.single_strand, dP, items, L.count, arena, dcodes, dw,
-                                      L.count <= ctx->eff_intra_wide);
+                                      L.count <= ctx->eff_intra_wide, ctx->intra_blocked != 0);
         });
     }
 }

@@ -98,6 +98,8 @@ struct bpmax_ctx {
                                          // 1024-lane (latency) form, larger ones the 512-lane
     int eff_intra_wide = 256;            // the current run's threshold
     int closure_ahead = 0;               // inner closure in one launch per diagonal (bpmax_tuning)
+    int intra_blocked = 1;               // F in-tile launches that are not wide use the blocked
+                                         // 256-lane form (0: the 512-lane push form)
     std::map<long long, bpx::Tuning> tuning; // tuned settings per problem class
     std::vector<hipEvent_t> sync_ev;
     bool graphs = false;

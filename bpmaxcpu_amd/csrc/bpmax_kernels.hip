@@ -314,7 +314,9 @@ __device__ void ext_acc512(const PairDesc &P, float *X, int I, int J, int sb, in
 // late (F, 512 lanes): the tile's two cross-tile steps that read logical diagonal J-I-1
 // (R1 with K = J-1, R2 with K' = I+1) are run here first, when the launch leaves them out of
 // the cross-tile items (see k_inner_closure).
-template <int T, bool MODE_S, int NT = kThreads>
+//
+// BLK (F, LT*LT/4 lanes): the blocked form below instead of the push wavefront.
+template <int T, bool MODE_S, int NT = kThreads, bool BLK = false>
 __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem,
                            bool late = false)
@@ -433,89 +435,160 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     const int LSL = fresh ? LX : L, LSR = fresh ? LX : L; // their row strides
     __syncthreads();
 
-    constexpr int NC = LT * LT / NT, QN = NT / LT; // cells per lane, lanes per column
-    static_assert(NT % LT == 0 && NC * QN == LT, "cell mapping");
-    const int b = tid % LT, q = tid / LT;
-    // per cell: running maximum, P2 weight (-inf where P2 does not apply, so that
-    // max(acc, p2 + w) needs no select), finalize step, P2 factor address
-    float acc[NC], wv[NC];
-    int sown[NC];
-    const float *p2p[NC];
+    if constexpr (BLK) {
+        // Blocked wavefront: lane (A, Bc) owns the 2 x 2 block of cells rows 2A..2A+1,
+        // columns 2Bc..2Bc+1.  Its cells lie on anti-diagonals s-1, s, s, s+1, and every
+        // in-tile factor of them lies in a block of a smaller block diagonal u = Bc - A or
+        // in the block itself, so the tile closes in one step per block diagonal (LT-1
+        // steps, LT/2 on a diagonal tile) instead of one per anti-diagonal:
+        //   step u: fold in the terms whose factor cells were finalized at step u-1 -- the
+        //     two of each cell's row in block (A, A+u-1) (R1) and the two of its column in
+        //     block (Bc-u+1, Bc) (R2), each with its S2 partner; at u = Bc - A close the
+        //     block in registers: BL, then TL and BR (each with the BL term), then TR.
+        // Factor cells outside the tile read the -inf pads (rows: the pad row LT; columns
+        // -LT..-1: the pad columns of the row above or the guard row); their partner
+        // indices are clamped into the tile's rows and columns.  On a diagonal tile the
+        // lower-left cell of a diagonal block is below the diagonal and stays -inf.
+        static_assert(!MODE_S && NT * 4 == LT * LT, "one 2 x 2 block per lane");
+        constexpr int HB = LT / 2;
+        const int A = tid / HB, Bc = tid % HB, a0 = 2 * A, b0 = 2 * Bc, town = Bc - A;
+        const bool active = !diag || A <= Bc;
+        float acc[2][2], wv[2][2];
 #pragma unroll
-    for (int m = 0; m < NC; ++m) {
-        const int a = q + m * QN;
-        acc[m] = Xs[a * LX + b];
-        const bool ok = (J * LT + b) - (I * LT + a) >= 4;
-        wv[m] = ok ? wl[5 * cr[a] + cc[b]] : NEG_INF; // P2 weight intra(b[I*LT+a], b[J*LT+b])
-        sown[m] = b + (LT - 1) - a;
-        const int row = a + 1, col = b - 1; // P2 factor X[a+1][b-1], or across the tile edge
-        p2p[m] = row < LT ? (col >= 0 ? Xs + row * LX + col : left + row) : (col >= 0 ? below + col : cornerp);
-    }
-    const int s0 = diag ? LT - 1 : 0;
-    // cells of anti-diagonal s0 start final
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int m = 0; m < NC; ++m)
-        if (s0 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], *p2p[m] + wv[m]);
-    // the P2 factor read at step s serves the finalize of step s+1: it belongs to the
-    // lane's cell on anti-diagonal s+2 (if any) and lies on anti-diagonal s
-    auto p2_next = [&](int s) -> const float * {
-        const int d = sown[0] - (s + 2); // that cell is m = d / QN
-        const float *p = cornerp;
-#pragma unroll
-        for (int m = 0; m < NC; ++m) p = d == m * QN ? p2p[m] : p;
-        return p;
-    };
-    float p2 = *p2_next(s0 - 1);
-    if constexpr (!MODE_S) {
-        // F tiles: S2 partners at constant stride L.  Per step one pointer per factor kind
-        // moves by a constant; cells of one lane sit QN rows apart (immediate offsets).
-        const float *pxr = Xs + q * LX + q - (LT - 1) + s0;           // X[q][q+s-(LT-1)]
-        const float *psr = SR + (q + s0 - (LT - 2)) * L + b;          // SR[q+s-(LT-2)][b]
-        const float *psl = SL + q * L + b - s0 + (LT - 2);            // SL[q][b-s+(LT-2)]
-        for (int s = s0; s < 2 * LT - 2; ++s) {
-            __syncthreads(); // anti-diagonal s is final
-            const float xcol = Xs[min(b - s + (LT - 1), LT) * LX + b]; // past the last row: pad row LT
-            float xr[NC], sr[NC], sl[NC];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                xr[m] = pxr[m * QN * (LX + 1)];
-                sr[m] = psr[m * QN * L];
-                sl[m] = psl[m * QN * L];
+            for (int j = 0; j < 2; ++j) {
+                const int a = a0 + i, c = b0 + j;
+                acc[i][j] = Xs[a * LX + c];
+                const bool ok = (J * LT + c) - (I * LT + a) >= 4;
+                wv[i][j] = ok ? wl[5 * cr[a] + cc[c]] : NEG_INF;
             }
-            const float p2n = *p2_next(s);
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
-                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], p2 + wv[m]);
+        // P2 factors X[a+1][b-1] of TL, BL, BR (TR's is BL), across the tile edge when outside
+        auto p2at = [&](int row, int col) -> const float * {
+            return row < LT ? (col >= 0 ? Xs + row * LX + col : left + row) : (col >= 0 ? below + col : cornerp);
+        };
+        const float *pTL = p2at(a0 + 1, b0 - 1), *pBL = p2at(a0 + 2, b0 - 1), *pBR = p2at(a0 + 2, b0);
+        const float sl_in = SL[a0 * L + a0], sr_in = SR[(b0 + 1) * L + b0 + 1]; // partners inside the block
+        const int u0 = diag ? 0 : 1 - HB;
+        for (int u = u0; u < HB; ++u) {
+            __syncthreads(); // block diagonal u-1 is final
+            if (!active || u > town) continue;
+            if (u > u0) {
+                const int k0 = 2 * (A + u - 1);  // row factors: columns k0, k0+1 (>= -LT)
+                const int k1 = 2 * (Bc - u + 1); // column factors: rows k1, k1+1 (> a0+1)
+                const float2 xr0 = *(const float2 *)(Xs + a0 * LX + k0);
+                const float2 xr1 = *(const float2 *)(Xs + (a0 + 1) * LX + k0);
+                const int rs = clampi(k0 + 1, 0, LT - 1);
+                const float2 sr0 = *(const float2 *)(SR + rs * L + b0);       // SR[k0+1][b0..]
+                const float2 sr1 = *(const float2 *)(SR + (rs + 1) * L + b0); // SR[k0+2][b0..]
+                const float2 xc0 = *(const float2 *)(Xs + min(k1, LT) * LX + b0);
+                const float2 xc1 = *(const float2 *)(Xs + min(k1 + 1, LT) * LX + b0);
+                const int cs = min(k1 - 1, L - 2);
+                const float sl00 = SL[a0 * L + cs], sl01 = SL[a0 * L + cs + 1];
+                const float sl10 = SL[(a0 + 1) * L + cs], sl11 = SL[(a0 + 1) * L + cs + 1];
+                max3_into(acc[0][0], xr0.x + sr0.x, xr0.y + sr1.x);
+                max3_into(acc[0][1], xr0.x + sr0.y, xr0.y + sr1.y);
+                max3_into(acc[1][0], xr1.x + sr0.x, xr1.y + sr1.x);
+                max3_into(acc[1][1], xr1.x + sr0.y, xr1.y + sr1.y);
+                max3_into(acc[0][0], sl00 + xc0.x, sl01 + xc1.x);
+                max3_into(acc[0][1], sl00 + xc0.y, sl01 + xc1.y);
+                max3_into(acc[1][0], sl10 + xc0.x, sl11 + xc1.x);
+                max3_into(acc[1][1], sl10 + xc0.y, sl11 + xc1.y);
             }
-            p2 = p2n;
-            pxr += 1;
-            psr += L;
-            psl -= 1;
+            if (u == town) {
+                const float xbl = fmaxf(acc[1][0], *pBL + wv[1][0]);
+                const float xtl = fmaxf(max3(acc[0][0], sl_in + xbl, NEG_INF), *pTL + wv[0][0]);
+                const float xbr = fmaxf(max3(acc[1][1], xbl + sr_in, NEG_INF), *pBR + wv[1][1]);
+                const float xtr = fmaxf(max3(acc[0][1], xtl + sr_in, sl_in + xbr), xbl + wv[0][1]);
+                *(float2 *)(Xs + a0 * LX + b0) = make_float2(xtl, xtr);
+                *(float2 *)(Xs + (a0 + 1) * LX + b0) = make_float2(xbl, xbr);
+            }
         }
     } else {
-        // S2 tiles: partners may be cells of the tile itself (fresh), read after the
-        // barrier and with clamped indices
-        for (int s = s0; s < 2 * LT - 2; ++s) {
-            __syncthreads(); // anti-diagonal s is final
-            const int ac = b - s + (LT - 1);
-            const float xcol = Xs[min(ac, LT) * LX + b];
-            const int ksl = ac - 1 < 0 ? 0 : ac - 1;
-            float xr[NC], sr[NC], sl[NC];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                const int a = q + m * QN, kr = a + s - (LT - 1);
-                xr[m] = Xs[a * LX + kr];
-                sr[m] = SR[clampi(kr + 1, 0, LT) * LSR + b];
-                sl[m] = SL[a * LSL + ksl];
+        constexpr int NC = LT * LT / NT, QN = NT / LT; // cells per lane, lanes per column
+        static_assert(NT % LT == 0 && NC * QN == LT, "cell mapping");
+        const int b = tid % LT, q = tid / LT;
+        // per cell: running maximum, P2 weight (-inf where P2 does not apply, so that
+        // max(acc, p2 + w) needs no select), finalize step, P2 factor address
+        float acc[NC], wv[NC];
+        int sown[NC];
+        const float *p2p[NC];
+    #pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            const int a = q + m * QN;
+            acc[m] = Xs[a * LX + b];
+            const bool ok = (J * LT + b) - (I * LT + a) >= 4;
+            wv[m] = ok ? wl[5 * cr[a] + cc[b]] : NEG_INF; // P2 weight intra(b[I*LT+a], b[J*LT+b])
+            sown[m] = b + (LT - 1) - a;
+            const int row = a + 1, col = b - 1; // P2 factor X[a+1][b-1], or across the tile edge
+            p2p[m] = row < LT ? (col >= 0 ? Xs + row * LX + col : left + row) : (col >= 0 ? below + col : cornerp);
+        }
+        const int s0 = diag ? LT - 1 : 0;
+        // cells of anti-diagonal s0 start final
+    #pragma unroll
+        for (int m = 0; m < NC; ++m)
+            if (s0 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], *p2p[m] + wv[m]);
+        // the P2 factor read at step s serves the finalize of step s+1: it belongs to the
+        // lane's cell on anti-diagonal s+2 (if any) and lies on anti-diagonal s
+        auto p2_next = [&](int s) -> const float * {
+            const int d = sown[0] - (s + 2); // that cell is m = d / QN
+            const float *p = cornerp;
+    #pragma unroll
+            for (int m = 0; m < NC; ++m) p = d == m * QN ? p2p[m] : p;
+            return p;
+        };
+        float p2 = *p2_next(s0 - 1);
+        if constexpr (!MODE_S) {
+            // F tiles: S2 partners at constant stride L.  Per step one pointer per factor kind
+            // moves by a constant; cells of one lane sit QN rows apart (immediate offsets).
+            const float *pxr = Xs + q * LX + q - (LT - 1) + s0;           // X[q][q+s-(LT-1)]
+            const float *psr = SR + (q + s0 - (LT - 2)) * L + b;          // SR[q+s-(LT-2)][b]
+            const float *psl = SL + q * L + b - s0 + (LT - 2);            // SL[q][b-s+(LT-2)]
+            for (int s = s0; s < 2 * LT - 2; ++s) {
+                __syncthreads(); // anti-diagonal s is final
+                const float xcol = Xs[min(b - s + (LT - 1), LT) * LX + b]; // past the last row: pad row LT
+                float xr[NC], sr[NC], sl[NC];
+    #pragma unroll
+                for (int m = 0; m < NC; ++m) {
+                    xr[m] = pxr[m * QN * (LX + 1)];
+                    sr[m] = psr[m * QN * L];
+                    sl[m] = psl[m * QN * L];
+                }
+                const float p2n = *p2_next(s);
+    #pragma unroll
+                for (int m = 0; m < NC; ++m) {
+                    max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
+                    if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], p2 + wv[m]);
+                }
+                p2 = p2n;
+                pxr += 1;
+                psr += L;
+                psl -= 1;
             }
-            const float p2n = *p2_next(s);
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-                max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
-                if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], p2 + wv[m]);
+        } else {
+            // S2 tiles: partners may be cells of the tile itself (fresh), read after the
+            // barrier and with clamped indices
+            for (int s = s0; s < 2 * LT - 2; ++s) {
+                __syncthreads(); // anti-diagonal s is final
+                const int ac = b - s + (LT - 1);
+                const float xcol = Xs[min(ac, LT) * LX + b];
+                const int ksl = ac - 1 < 0 ? 0 : ac - 1;
+                float xr[NC], sr[NC], sl[NC];
+    #pragma unroll
+                for (int m = 0; m < NC; ++m) {
+                    const int a = q + m * QN, kr = a + s - (LT - 1);
+                    xr[m] = Xs[a * LX + kr];
+                    sr[m] = SR[clampi(kr + 1, 0, LT) * LSR + b];
+                    sl[m] = SL[a * LSL + ksl];
+                }
+                const float p2n = *p2_next(s);
+    #pragma unroll
+                for (int m = 0; m < NC; ++m) {
+                    max3_into(acc[m], xr[m] + sr[m], sl[m] + xcol);
+                    if (s + 1 == sown[m]) Xs[(q + m * QN) * LX + b] = fmaxf(acc[m], p2 + wv[m]);
+                }
+                p2 = p2n;
             }
-            p2 = p2n;
         }
     }
     __syncthreads();
@@ -592,6 +665,20 @@ __global__ __launch_bounds__(NT, MODE_S ? 1 : 8) void k_inner_intra(const PairDe
     __builtin_amdgcn_s_setprio(3);
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     intra_body<T, MODE_S, NT>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
+}
+
+// In-tile closure of F in the blocked form (one 2 x 2 block of cells per lane, 256 lanes).
+template <int T>
+__global__ __launch_bounds__(kThreads, 8) void k_inner_intra_blk(const PairDesc *__restrict__ pairs,
+                                                              const WorkItem *__restrict__ items,
+                                                              float *__restrict__ arena,
+                                                              const int *__restrict__ codes,
+                                                              const float *__restrict__ w)
+{
+    __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
+    __builtin_amdgcn_s_setprio(3);
+    const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
+    intra_body<T, false, kThreads, true>(pairs[it.pair], it.i1, it.j1, it.I, it.J, arena, codes, w, smem);
 }
 
 // The inner closure of F by logical block diagonal DL in one launch per diagonal instead of
@@ -1005,9 +1092,15 @@ hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const Pai
 }
 
 hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
-                              const WorkItem *items, int n, float *arena, const int *codes, const float *w, bool wide)
+                              const WorkItem *items, int n, float *arena, const int *codes, const float *w, bool wide,
+                              bool blocked)
 {
     if (n <= 0) return hipSuccess;
+    if (blocked && !single_strand && !wide) {
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_inner_intra_blk<TT>, dim3(n), dim3(kThreads), 0, st, pairs, items,
+                                             arena, codes, w));
+        return hipGetLastError();
+    }
 #define BPX_INTRA(S, NT) \
     BPX_T_DISPATCH(T, hipLaunchKernelGGL((k_inner_intra<TT, S, NT>), dim3(n), dim3(NT), 0, st, pairs, items, arena, codes, w))
     if (single_strand) {

@@ -9,10 +9,11 @@ hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const Work
                         float *arena, const int *codes, const float *w);
 hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                             const WorkItem *items, int n, float *arena);
-// wide: 1024-lane workgroups (latency) instead of 256 (throughput)
+// F: wide = 1024-lane workgroups (one cell per lane), else 512 (two cells per lane) or, with
+// blocked, 256 (one 2 x 2 block per lane, one step per block diagonal); S2: 1024 or 256
 hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                               const WorkItem *items, int n, float *arena, const int *codes, const float *w,
-                              bool wide);
+                              bool wide, bool blocked);
 // in-tile closure of one logical diagonal with the next diagonal's early cross-tile steps
 // (items typed TASK_INTRA / TASK_EXT), 512-lane workgroups
 hipError_t launch_inner_closure(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n,

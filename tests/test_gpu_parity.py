@@ -31,15 +31,24 @@ def eng():
 
 
 @pytest.fixture(scope="module",
-                params=["pipe", "pipe-eager", "pipe-ahead", "launch-narrow", "launch-wide", "launch-ahead", "tasks"])
+                params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "launch-narrow", "launch-wide", "launch-ahead",
+                        "tasks"])
 def eng2(request):
     """An engine under each schedule: two-stream pipeline (graph-replayed and eager; R0 work
-    items of one K each), per-phase launches on one stream with the in-tile closure always
-    512-lane (narrow; R0 items over every K of a tile) or always 1024-lane (wide; R0 items of
-    two K each, the last one ragged), each of the two with the inner closure in one launch per
-    diagonal (ahead: cross-tile steps carried by the previous diagonal's launch and by the
-    in-tile workgroups; graph-replayed in the pipeline), and the dependency-tile scheduler."""
-    e = B.Engine(0)
+    items of one K each), per-phase launches on one stream with the in-tile closure always in
+    the blocked 256-lane form (narrow; R0 items over every K of a tile) or always 1024-lane
+    (wide; R0 items of two K each, the last one ragged), each of the two with the inner
+    closure in one launch per diagonal (ahead: cross-tile steps carried by the previous
+    diagonal's launch and by the in-tile workgroups; graph-replayed in the pipeline), the
+    pipeline with every in-tile launch in the 512-lane push form (push), and the
+    dependency-tile scheduler."""
+    push = request.param.endswith("push")
+    if push:  # read when the context is created
+        os.environ["BPMAX_INTRA_BLOCKED"] = "0"
+    try:
+        e = B.Engine(0)
+    finally:
+        os.environ.pop("BPMAX_INTRA_BLOCKED", None)
     sched = request.param.split("-")[0]
     ahead = request.param.endswith("ahead")
     if sched == "launch":
@@ -47,9 +56,9 @@ def eng2(request):
         e.set_default_tuning({"schedule": "launch", "outer_split": 1, "outer_items": 2048, "graphs": False,
                               "intra_wide": 0 if narrow else 1 << 30, "outer_kgroup": 0 if narrow else 2,
                               "closure_ahead": ahead})
-    elif ahead:
-        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": True,
-                              "intra_wide": 256, "outer_kgroup": 1, "closure_ahead": True})
+    elif ahead or push:
+        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": ahead,
+                              "intra_wide": 256 if ahead else 0, "outer_kgroup": 1, "closure_ahead": ahead})
     e.set_schedule(sched)
     e.set_graphs(request.param in ("pipe", "pipe-ahead"))
     yield e
