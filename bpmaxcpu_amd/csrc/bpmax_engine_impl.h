@@ -95,7 +95,8 @@ struct bpmax_ctx {
     int outer_items = 2048;              // work items an R0 launch aims for (split-K granularity)
     int outer_kgroup = 1;                // K values per R0 work item (0: every K of the tile)
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
-                                         // 1024-lane (latency) form, larger ones the 512-lane
+                                         // 1024-lane (latency) form, larger ones the blocked
+                                         // (or, with intra_blocked = 0, the 512-lane) form
     int eff_intra_wide = 256;            // the current run's threshold
     int closure_ahead = 0;               // inner closure in one launch per diagonal (bpmax_tuning)
     int intra_blocked = 1;               // F in-tile launches that are not wide use the blocked

@@ -637,8 +637,9 @@ __global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__
 
 // The closure kernels (cross-tile and in-tile) are the critical path of the pipelined
 // schedule and run beside its R0 launches.  They fit on a CU next to four R0 workgroups
-// (at most 64 VGPRs per SIMD lane between them, i.e. 8 waves of up to 64 or 2 of up to
-// 32 each, and 9 / 19 KB of LDS; the 512-lane in-tile form for F) and raise their wave
+// (at most 64 VGPRs per SIMD lane between them: 4 waves of up to 64 -- the cross-tile
+// kernel, the blocked in-tile form -- or 8 of up to 32 -- the 512-lane in-tile forms; 9 /
+// 19 KB of LDS) and raise their wave
 // priority, so they start at once and issue ahead of the R0 waves instead of queueing
 // behind them.
 template <int T, bool MODE_S>

@@ -238,8 +238,10 @@ typedef struct {
     int outer_items; /* work items an R0 launch aims for, >= 64 (split-K granularity)   */
     int graphs;      /* HIP-graph replay of the pipelined schedule                      */
     int intra_wide;  /* in-tile closure launches of at most this many tiles use 1024-lane
-                        workgroups (one cell per lane: latency), larger ones 512-lane
-                        workgroups (two cells per lane: throughput)                     */
+                        workgroups (one cell per lane: latency), larger ones 256-lane
+                        workgroups (one 2x2 block of cells per lane, one step per block
+                        diagonal; 512-lane, two cells per lane, with
+                        BPMAX_INTRA_BLOCKED=0)                                            */
     int outer_kgroup; /* K values (storage tiles of the inner sum) per R0 work item; 0:
                         all of an output tile's.  1 keeps the items of one triangle in
                         step, so each source tile is read from HBM once per k1 and then
