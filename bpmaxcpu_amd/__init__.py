@@ -126,7 +126,7 @@ class Tuning(ctypes.Structure):
     def from_dict(tuning: dict) -> "Tuning":
         return Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
                       1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256), tuning.get("outer_kgroup", 1),
-                      1 if tuning.get("closure_ahead") else 0)
+                      1 if tuning.get("closure_ahead", True) else 0)
 
 
 def schedule_check(shapes, tuning: dict, skip: int = -1):

@@ -520,7 +520,7 @@ static int pick_chunk(long long total_steps, long long tiles)
 //   default one launch per storage diagonal D: all storage tiles (I,I+D) of every triangle
 //   on d1; outer_split > 0 merges them into that many ranges), then the inner closure by
 //   logical block diagonal DL (cross-tile launch, closure launch) -- or, with
-//   closure_ahead, one launch per DL (k_inner_closure): the in-tile items of DL, whose
+//   closure_ahead, one launch per DL (k_inner_closure_blk): the in-tile items of DL, whose
 //   workgroups first run their tiles' cross-tile steps D-1 and D (the two that read
 //   diagonal DL-1), and cross-tile items with the other steps of the tiles on DL+1 (they
 //   read diagonals below DL only).
@@ -659,9 +659,9 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
                     for (size_t p = 0; p < P.size(); ++p)
                         if (P[p].T == T && P[p].M > d1 && P[p].nbl > D1)
                             tot += (long long)(P[p].M - d1) * (P[p].nbl - D1) * 2 * D;
-                    // longer items than the cross-tile launches': a 512-lane item's fixed cost
-                    // (staging latency, workgroup start) is large next to its lean steps
-                    // (64x512: 8 steps 70.8 ms, 2 steps 86.5 ms; 24x512: 8 steps 11.8 ms, 5 steps 14.2 ms)
+                    // longer items than the cross-tile launches' (these share launches with
+                    // the in-tile work; 64x512, 24x512 and 48x384 ran faster with 8..32 steps
+                    // than with 4 or with the cross-tile launches' chunks)
                     const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
                     const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
                     for (size_t p = 0; p < P.size() && D > 0; ++p) {

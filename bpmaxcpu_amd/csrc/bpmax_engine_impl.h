@@ -26,7 +26,7 @@ struct Launch {
     int d1;              // outer diagonal (-1: S2 phases)
     int diag;            // storage block diagonal (outer) or logical block diagonal (inner)
     bool ahead = false;  // in-tile launch that also carries diagonal diag+1's early
-                         // cross-tile steps (k_inner_closure)
+                         // cross-tile steps (k_inner_closure_blk)
 };
 
 // launch-schedule settings of one problem class (bpmax_tuning)
@@ -98,7 +98,7 @@ struct bpmax_ctx {
                                          // 1024-lane (latency) form, larger ones the blocked
                                          // (or, with intra_blocked = 0, the 512-lane) form
     int eff_intra_wide = 256;            // the current run's threshold
-    int closure_ahead = 0;               // inner closure in one launch per diagonal (bpmax_tuning)
+    int closure_ahead = 1;               // inner closure in one launch per diagonal (bpmax_tuning)
     int intra_blocked = 1;               // F in-tile launches that are not wide use the blocked
                                          // 256-lane form (0: the 512-lane push form)
     std::map<long long, bpx::Tuning> tuning; // tuned settings per problem class

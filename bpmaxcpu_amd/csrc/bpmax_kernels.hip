@@ -163,23 +163,19 @@ __device__ __forceinline__ ExtStep ext_step(int I, int J, int s)
     return ExtStep{K2, -1, K2, 0, false, true, false, false, K2 == J};
 }
 
-// Always max-combined into the tile (other items of the same tile run concurrently).
+// Steps [sb, se) of the cross-tile part of logical tile (I,J) of X on 256 lanes, folded
+// into acc (this lane's R x R block at lane_block).
 template <int T, bool MODE_S>
-__device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
+__device__ void ext_accumulate(const PairDesc &P, float *X, int I, int J, int sb, int se,
+                               float (&acc)[Tc<Inner<T>::LT>::R][Tc<Inner<T>::LT>::R], float *arena, float *smem)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R;
+    constexpr int LD = Tc<LT>::LD;
     float *As = smem, *Bs = smem + LT * LD;
     int r0, c0;
     lane_block<LT>(r0, c0);
-    const int I = it.I, J = it.J, nb = P.nb;
+    const int nb = P.nb;
     float *S = arena + P.S2;
-    float *X = MODE_S ? S : tri_at(arena, P, it.i1, it.j1);
-    float acc[R][R];
-#pragma unroll
-    for (int i = 0; i < R; ++i)
-#pragma unroll
-        for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
     FetchA<LT> fa;
     FetchB<LT> fb;
     ExtStep st{};
@@ -188,99 +184,35 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
         fetch_A<T, LT>(fa, st.a_x ? X : S, nb, I, st.aK, st.cshift);
         fetch_B<T, LT>(fb, st.b_x ? X : S, nb, st.bK, J, st.rshift);
     };
-    if (it.step_begin < it.step_end) fetch(it.step_begin);
-    for (int s = it.step_begin; s < it.step_end; ++s) {
+    if (sb < se) fetch(sb);
+    for (int s = sb; s < se; ++s) {
         __syncthreads();
         commit_A<LT>(As, fa, st.cshift, st.lastcol);
         commit_B<LT>(Bs, fb);
         int kb, ke;
         quad_k_range<LT>(st.adiag, st.bdiag, st.lastcol, kb, ke);
         __syncthreads();
-        if (s + 1 < it.step_end) fetch(s + 1);
+        if (s + 1 < se) fetch(s + 1);
         product<LT>(acc, As, Bs, r0, c0, kb, ke);
     }
-    store_acc<T, LT>(ltile<T, LT>(X, I, J, nb), acc, r0, c0, true);
 }
 
-// Steps [sb, se) of the cross-tile part of logical tile (I,J) of X = F(i1,j1) on 512 lanes
-// (two 256-lane halves, one staging buffer pair), lean enough for the 32 VGPRs a 512-lane
-// closure workgroup may use beside four R0 workgroups:
-//   * the lower half stages each step's A operand, row-major (As[r][k], k shifted by
-//     -cshift), the upper half its B operand (Bs[k][c]): one float4 per lane;
-//   * the halves take the lower and the upper half of the step's k range (quad_k_range
-//     gives multiples of QS = 16), one k per loop round (few live registers).
-// acc: this lane's 2 x 2 block maxima over its half of k.
-template <int T>
-__device__ void ext_acc512(const PairDesc &P, float *X, int I, int J, int sb, int se,
-                           float (&acc)[Tc<Inner<T>::LT>::R][Tc<Inner<T>::LT>::R], float *arena, float *smem)
+// Always max-combined into the tile (other items of the same tile run concurrently).
+template <int T, bool MODE_S>
+__device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int LD = Tc<LT>::LD, R = Tc<LT>::R, Q = Stage<LT>::Q;
-    static_assert(Stage<LT>::PER == 1 && R == 2 && LD % 4 == 0, "one staged float4 per lane, 2 x 2 blocks");
-    float *As = smem, *Bs = smem + LT * LD;
-    float *S = arena + P.S2;
-    const int nbl = P.nb * (T / LT);
-    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x / kThreads), tid = threadIdx.x % kThreads;
+    constexpr int R = Tc<LT>::R;
+    float *X = MODE_S ? arena + P.S2 : tri_at(arena, P, it.i1, it.j1);
+    float acc[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
+    ext_accumulate<T, MODE_S>(P, X, it.I, it.J, it.step_begin, it.step_end, acc, arena, smem);
     int r0, c0;
-    lane_block<LT>(r0, c0, tid);
-    const int sr = tid / Q, sc4 = (tid % Q) * 4; // staged row and first column (A: row r, B: row k)
-    float4 v = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
-    float prev = NEG_INF; // A with column shift -1: X[I*LT + r][aK*LT - 1], lanes r < LT
-    ExtStep st{};
-    auto fetch = [&](int s) {
-        st = ext_step<false>(I, J, s);
-        // tile bases are uniform; only B's last row can cross into the next tile
-        if (half == 0) {
-            const float *A = st.a_x ? X : S;
-            const bool have = st.aK >= I && st.aK < nbl;
-            const float4 x = *(const float4 *)(ltile<T, LT>(A, I, have ? st.aK : I, P.nb) + sr * T + sc4);
-            v = have ? x : make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
-            prev = NEG_INF;
-            if (st.cshift < 0 && st.aK - 1 >= I && tid < LT)
-                prev = ltile<T, LT>(A, I, st.aK - 1, P.nb)[tid * T + LT - 1];
-        } else {
-            const float *B = st.b_x ? X : S;
-            const bool ok1 = st.bK + 1 <= J && st.bK + 1 < nbl;
-            const float *t0 = ltile<T, LT>(B, st.bK, J, P.nb), *t1 = ltile<T, LT>(B, ok1 ? st.bK + 1 : J, J, P.nb);
-            const int row = sr + st.rshift;
-            const bool cross = row >= LT;
-            const float4 y = *(const float4 *)((cross ? t1 : t0 + row * T) + sc4);
-            v = cross && !ok1 ? make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF) : y;
-        }
-    };
-    if (sb < se) fetch(sb);
-    for (int s = sb; s < se; ++s) {
-        __syncthreads();
-        if (half == 0) {
-            float *row = As + sr * LD + sc4;
-            if (st.cshift == 0) {
-                *(float4 *)row = v;
-            } else { // As[r][k] = A[r][k-1]: columns shift right by one, column 0 from prev
-                row[1] = v.x;
-                row[2] = v.y;
-                row[3] = v.z;
-                if (sc4 + 4 < LT) row[4] = v.w;
-                if (tid < LT) As[tid * LD] = prev;
-            }
-        } else {
-            *(float4 *)(Bs + sr * LD + sc4) = v;
-        }
-        int kb, ke;
-        quad_k_range<LT>(st.adiag, st.bdiag, false, kb, ke, tid);
-        const int mid = kb + ((ke - kb) >> 1);
-        const int k0 = half ? mid : kb, k1 = half ? ke : mid;
-        __syncthreads();
-        if (s + 1 < se) fetch(s + 1);
-#pragma unroll 1
-        for (int k = k0; k < k1; ++k) {
-            const float a0 = As[r0 * LD + k], a1 = As[(r0 + 1) * LD + k];
-            const float2 b = *(const float2 *)(Bs + k * LD + c0);
-            acc[0][0] = fmaxf(acc[0][0], a0 + b.x);
-            acc[0][1] = fmaxf(acc[0][1], a0 + b.y);
-            acc[1][0] = fmaxf(acc[1][0], a1 + b.x);
-            acc[1][1] = fmaxf(acc[1][1], a1 + b.y);
-        }
-    }
+    lane_block<LT>(r0, c0);
+    store_acc<T, LT>(ltile<T, LT>(X, it.I, it.J, P.nb), acc, r0, c0, true);
 }
 
 // ---------------------------------------------------------------------- inner intra
@@ -311,9 +243,9 @@ __device__ void ext_acc512(const PairDesc &P, float *X, int I, int J, int sb, in
 // All LDS reads of a step are conflict-free: row factors are broadcasts, column factors
 // and S2 partners run along consecutive columns.
 //
-// late (F, 512 lanes): the tile's two cross-tile steps that read logical diagonal J-I-1
+// late (blocked form only): the tile's two cross-tile steps that read logical diagonal J-I-1
 // (R1 with K = J-1, R2 with K' = I+1) are run here first, when the launch leaves them out of
-// the cross-tile items (see k_inner_closure).
+// the cross-tile items (see k_inner_closure_blk).
 //
 // BLK (F, LT*LT/4 lanes): the blocked form below instead of the push wavefront.
 template <int T, bool MODE_S, int NT = kThreads, bool BLK = false>
@@ -336,7 +268,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
 
     // late cross-tile steps: staged in the LDS that the set-up below overwrites, results
     // kept in registers until the tile is in LDS
-    constexpr bool kLate = !MODE_S && NT == 2 * kThreads;
+    constexpr bool kLate = !MODE_S && BLK;
     constexpr int RX = Tc<LT>::R;
     float eacc[RX][RX];
     const bool late_steps = kLate && late && J > I;
@@ -346,7 +278,7 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
             for (int i = 0; i < RX; ++i)
 #pragma unroll
                 for (int j = 0; j < RX; ++j) eacc[i][j] = NEG_INF;
-            ext_acc512<T>(P, X, I, J, J - I - 1, J - I + 1, eacc, arena, smem);
+            ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, eacc, arena, smem);
             __syncthreads();
         }
     }
@@ -411,20 +343,17 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     if (tid == 0) cornerp[0] = cn;
     if (tid < 25) wl[tid] = vw;
     if constexpr (kLate) {
-        if (late_steps) { // fold the late steps into the tile, one 256-lane half at a time
+        if (late_steps) { // fold the late steps into the tile (each cell has one owner)
             int er0, ec0;
-            lane_block<LT>(er0, ec0, tid % kThreads);
-            for (int h = 0; h < 2; ++h) {
-                __syncthreads();
-                if (tid / kThreads == h)
+            lane_block<LT>(er0, ec0);
+            __syncthreads();
 #pragma unroll
-                    for (int i = 0; i < RX; ++i)
+            for (int i = 0; i < RX; ++i)
 #pragma unroll
-                        for (int j = 0; j < RX; ++j) {
-                            float *c = Xs + (er0 + i) * LX + ec0 + j;
-                            *c = fmaxf(*c, eacc[i][j]);
-                        }
-            }
+                for (int j = 0; j < RX; ++j) {
+                    float *c = Xs + (er0 + i) * LX + ec0 + j;
+                    *c = fmaxf(*c, eacc[i][j]);
+                }
         }
     }
     // SL = S2 (I,I), SR = S2 (J,J) (both in Sb on a diagonal tile); S2's own diagonal tiles
@@ -688,41 +617,24 @@ __global__ __launch_bounds__(kThreads, 8) void k_inner_intra_blk(const PairDesc 
 //     cross-tile steps of its tile that read diagonal DL-1 (intra_body, late), and
 //   * cross-tile items (type TASK_EXT) with the other steps of the tiles on DL+1, which
 //     read diagonals below DL only and so run beside the in-tile work of DL.
-// 512 lanes per workgroup, at most 32 VGPRs each (the bound below asks for that), so that
-// like k_inner_intra it fits beside four R0 workgroups.  (The bound's occupancy target of
-// 16 waves per SIMD is not met -- SGPRs limit it -- and need not be: only the VGPR budget
-// matters for running beside R0.)
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Wpass-failed"
+// 256 lanes: the blocked in-tile form and the cross-tile body; fits beside four R0
+// workgroups like k_inner_ext.
 template <int T>
-__global__ __launch_bounds__(2 * kThreads, 16) void k_inner_closure(const PairDesc *__restrict__ pairs,
+__global__ __launch_bounds__(kThreads, 8) void k_inner_closure_blk(const PairDesc *__restrict__ pairs,
                                                                 const WorkItem *__restrict__ items,
                                                                 float *__restrict__ arena,
                                                                 const int *__restrict__ codes,
                                                                 const float *__restrict__ w)
 {
-    constexpr int LT = Inner<T>::LT, R = Tc<LT>::R;
-    static_assert(Tc<LT>::PROD_FLOATS <= Intra<LT>::FLOATS, "staging fits the in-tile LDS");
-    __shared__ __attribute__((aligned(16))) float smem[Intra<LT>::FLOATS];
+    static_assert(Tc<Inner<T>::LT>::PROD_FLOATS <= Intra<Inner<T>::LT>::FLOATS, "staging fits the in-tile LDS");
+    __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
     __builtin_amdgcn_s_setprio(3);
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     const PairDesc &P = pairs[it.pair];
-    if (wi_type(it.flags) == TASK_EXT) {
-        float *X = tri_at(arena, P, it.i1, it.j1);
-        float acc[R][R];
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-#pragma unroll
-            for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
-        ext_acc512<T>(P, X, it.I, it.J, it.step_begin, it.step_end, acc, arena, smem);
-        int r0, c0;
-        lane_block<LT>(r0, c0, threadIdx.x % kThreads);
-        store_acc<T, LT>(ltile<T, LT>(X, it.I, it.J, P.nb), acc, r0, c0, true);
-    } else {
-        intra_body<T, false, 2 * kThreads>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem, true);
-    }
+    if (wi_type(it.flags) == TASK_EXT) ext_body<T, false>(P, it, arena, smem);
+    else intra_body<T, false, kThreads, true>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem, true);
 }
-#pragma clang diagnostic pop
+
 
 __global__ __launch_bounds__(kThreads) void k_s1(const PairDesc *__restrict__ pairs, float *__restrict__ arena,
                                                  const int *__restrict__ codes, const float *__restrict__ w)
@@ -1117,7 +1029,7 @@ hipError_t launch_inner_closure(hipStream_t st, int T, const PairDesc *pairs, co
                                 float *arena, const int *codes, const float *w)
 {
     if (n <= 0) return hipSuccess;
-    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_inner_closure<TT>, dim3(n), dim3(2 * kThreads), 0, st, pairs, items, arena,
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_inner_closure_blk<TT>, dim3(n), dim3(kThreads), 0, st, pairs, items, arena,
                                          codes, w));
     return hipGetLastError();
 }

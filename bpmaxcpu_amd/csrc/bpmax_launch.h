@@ -14,8 +14,8 @@ hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const Pai
 hipError_t launch_inner_intra(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                               const WorkItem *items, int n, float *arena, const int *codes, const float *w,
                               bool wide, bool blocked);
-// in-tile closure of one logical diagonal with the next diagonal's early cross-tile steps
-// (items typed TASK_INTRA / TASK_EXT), 512-lane workgroups
+// in-tile closure of one logical diagonal (blocked form) with the next diagonal's early
+// cross-tile steps (items typed TASK_INTRA / TASK_EXT), 256-lane workgroups
 hipError_t launch_inner_closure(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n,
                                 float *arena, const int *codes, const float *w);
 hipError_t launch_tasks(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *tasks, int ntasks, int *head,

@@ -246,11 +246,12 @@ typedef struct {
                         all of an output tile's.  1 keeps the items of one triangle in
                         step, so each source tile is read from HBM once per k1 and then
                         served from L2 to all its consumers (default 1)                  */
-    int closure_ahead; /* 1: the inner closure of F runs as one launch per logical block
-                        diagonal DL, its in-tile workgroups also running their tiles' two
-                        cross-tile steps that read DL-1, and the other cross-tile steps of
-                        DL+1 riding in the same launch (half as many dependent launches);
-                        0: a cross-tile and an in-tile launch per diagonal               */
+    int closure_ahead; /* 1 (default): the inner closure of F runs as one launch per
+                        logical block diagonal DL, its in-tile workgroups (blocked form)
+                        also running their tiles' two cross-tile steps that read DL-1, and
+                        the other cross-tile steps of DL+1 riding in the same launch (half
+                        as many dependent launches); 0: a cross-tile and an in-tile launch
+                        per diagonal                                                     */
 } bpmax_tuning;
 /* Time every candidate setting on `npairs` synthetic M x N pairs (median of `reps` runs
  * after a warm-up each), keep the fastest for the class and return it. */

@@ -34,14 +34,17 @@ def eng():
                 params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "launch-narrow", "launch-wide", "launch-ahead",
                         "tasks"])
 def eng2(request):
-    """An engine under each schedule: two-stream pipeline (graph-replayed and eager; R0 work
-    items of one K each), per-phase launches on one stream with the in-tile closure always in
-    the blocked 256-lane form (narrow; R0 items over every K of a tile) or always 1024-lane
-    (wide; R0 items of two K each, the last one ragged), each of the two with the inner
-    closure in one launch per diagonal (ahead: cross-tile steps carried by the previous
-    diagonal's launch and by the in-tile workgroups; graph-replayed in the pipeline), the
-    pipeline with every in-tile launch in the 512-lane push form (push), and the
-    dependency-tile scheduler."""
+    """An engine under each schedule:
+    * pipe: the two-stream pipeline with the default settings (inner closure in one launch
+      per diagonal -- cross-tile steps carried by the previous diagonal's launch and by the
+      in-tile workgroups -- and R0 items of one K each), graph-replayed;
+    * pipe-eager: the pipeline with a cross-tile and an in-tile launch per diagonal, in-tile
+      launches 1024-lane up to 256 tiles, blocked above; pipe-ahead: the default closure,
+      eager; pipe-push: every in-tile launch in the 512-lane push form;
+    * launch-*: per-phase launches on one stream with the in-tile closure always blocked
+      (narrow; R0 items over every K of a tile), always 1024-lane (wide; R0 items of two K
+      each, the last one ragged) or in one launch per diagonal (ahead);
+    * tasks: the dependency-tile scheduler."""
     push = request.param.endswith("push")
     if push:  # read when the context is created
         os.environ["BPMAX_INTRA_BLOCKED"] = "0"
@@ -56,11 +59,11 @@ def eng2(request):
         e.set_default_tuning({"schedule": "launch", "outer_split": 1, "outer_items": 2048, "graphs": False,
                               "intra_wide": 0 if narrow else 1 << 30, "outer_kgroup": 0 if narrow else 2,
                               "closure_ahead": ahead})
-    elif ahead or push:
-        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": ahead,
-                              "intra_wide": 256 if ahead else 0, "outer_kgroup": 1, "closure_ahead": ahead})
+    elif sched == "pipe" and request.param != "pipe":
+        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": False,
+                              "intra_wide": 0 if push else 256, "outer_kgroup": 1, "closure_ahead": ahead})
     e.set_schedule(sched)
-    e.set_graphs(request.param in ("pipe", "pipe-ahead"))
+    e.set_graphs(request.param == "pipe")
     yield e
     e.close()
 
