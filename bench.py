@@ -247,9 +247,10 @@ def _pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else None
 
 
-def serving_leg(B, n_plain=40, n_sat=160, n_rate=160, fractions=(0.25, 0.5)):
+def serving_leg(B, n_plain=160, n_sat=160, n_rate=160, fractions=(0.25, 0.5)):
     """Continuous-batching server: saturated throughput and p50/p99 latency at Poisson
-    arrival rates below saturation, against the plain path (one solve call per pair)."""
+    arrival rates below saturation, against the plain path (one solve call per pair) on the
+    same requests."""
     reqs = serving_requests(max(n_plain, n_sat, n_rate))
     cells = [B.cell_updates(len(a), len(b)) for a, b in reqs]
     eng = B.Engine(0)
