@@ -76,8 +76,9 @@ def test_server_results_match_solo_solves():
 
 
 @pytest.mark.skip(reason="stopped making progress on the GPU in two runs (with a third inner stream, then with a "
-                         "one-wait-per-launch enqueue, both since removed); no cause found in the server or the "
-                         "pipelined enqueue, so the case is not run until one is")
+                         "one-wait-per-launch enqueue, both since removed); with the current enqueue a throughput "
+                         "probe with 3 slots and batches of <= 8 pairs stalled the same way; no cause found in the "
+                         "server or the pipelined enqueue, so the case is not run until one is")
 def test_server_overlaps_batches_and_respects_budget():
     """A budget of about three mid-size pairs: batches must run side by side, the arena
     in use never exceeds the budget, and every score still matches a solo solve."""
