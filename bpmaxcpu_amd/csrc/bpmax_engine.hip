@@ -703,31 +703,30 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
                 }
                 size_t f = S.items.size();
                 const int tin = closure_ahead ? TASK_INTRA << WI_TYPE_SHIFT : 0;
+                // closure ahead: steps [0, D) and [D+2, 2D+2) of the tiles on D+1, in items
+                // longer than the cross-tile launches' (these share launches with the in-tile
+                // work; 64x512, 24x512 and 48x384 ran faster with 8..32 steps than with 4 or
+                // with the cross-tile launches' chunks)
+                const int D1 = D + 1;
+                long long tot = 0;
+                for (size_t p = 0; p < P.size() && closure_ahead; ++p)
+                    if (P[p].T == T && P[p].M > d1 && P[p].nbl > D1)
+                        tot += (long long)(P[p].M - d1) * (P[p].nbl - D1) * 2 * D;
+                const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
+                const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
+                // by triangle, each triangle's in-tile items then its cross-tile items, so
+                // that every XCD's contiguous share of the launch mixes the two kinds
                 for (size_t p = 0; p < P.size(); ++p) {
                     if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D) continue;
-                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1) {
                         for (int I = 0; I + D < P[p].nbl; ++I)
                             S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, tin});
-                }
-                if (closure_ahead) { // steps [0, D) and [D+2, 2D+2) of the tiles on D+1
-                    const int D1 = D + 1;
-                    long long tot = 0;
-                    for (size_t p = 0; p < P.size(); ++p)
-                        if (P[p].T == T && P[p].M > d1 && P[p].nbl > D1)
-                            tot += (long long)(P[p].M - d1) * (P[p].nbl - D1) * 2 * D;
-                    // longer items than the cross-tile launches' (these share launches with
-                    // the in-tile work; 64x512, 24x512 and 48x384 ran faster with 8..32 steps
-                    // than with 4 or with the cross-tile launches' chunks)
-                    const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
-                    const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
-                    for (size_t p = 0; p < P.size() && D > 0; ++p) {
-                        if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D1) continue;
-                        for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                            for (int I = 0; I + D1 < P[p].nbl; ++I) {
-                                const WorkItem base{(int)p, i1, i1 + d1, I, I + D1, 0, 0, tex};
-                                chunked_range(S.items, base, 0, D, ch);
-                                chunked_range(S.items, base, D + 2, 2 * D1, ch);
-                            }
+                        if (!closure_ahead || D == 0) continue;
+                        for (int I = 0; I + D1 < P[p].nbl; ++I) {
+                            const WorkItem base{(int)p, i1, i1 + d1, I, I + D1, 0, 0, tex};
+                            chunked_range(S.items, base, 0, D, ch);
+                            chunked_range(S.items, base, D + 2, 2 * D1, ch);
+                        }
                     }
                 }
                 emit(BPMAX_K_INNER_INTRA, g, false, f, 0, d1, D);
