@@ -53,7 +53,9 @@ struct OuterLds {
     static constexpr int FLOATS = Tc<T>::PROD_FLOATS + Tc<T>::LD; // As, B's extra row, Bs
 };
 
-template <int T>
+// BAL: balance the waves' k ranges on diagonal steps (below); off in the task kernel, whose
+// register budget it would exceed.
+template <int T, bool BAL = true>
 __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
@@ -89,6 +91,39 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
         if (K == J) s1b_f = S1[(k1 + 1) * M + j1];
     };
     if (it.step_begin < it.step_end) fetch(it.step_begin);
+    // Items of one K value: every step has the same diagonal type.  On diagonal steps the
+    // quadrants' k ranges differ (quad_k_range), so a wave with a shorter range also runs a
+    // slice of a longer one of another wave's quadrant into acc2, which that wave max-merges
+    // at the end of the item: the slowest wave's k range per step drops from 2 QS to 1.5 QS
+    // (K = I or K = J) or QS (K = I = J).  Max is exact, so the split changes no result.
+    constexpr int QS = Tc<T>::QS, H = QS / 2;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), qr = wv >> 1, qc = wv & 1;
+    int mb = 0, me = T, hw = -1, hb = 0, he = 0; // own range; helped wave and its slice
+    bool helped = false;
+    const bool balance = BAL && nK == 1 && (Kb == I || Kb == J) && it.step_begin < it.step_end;
+    if (balance) {
+        if (Kb == I && Kb == J) { // w0 [0,QS)  w1 [0,2QS) -> w1 [QS,2QS) + w2 [0,QS)  w3 [QS,2QS)
+            mb = wv == 3 || wv == 1 ? QS : 0;
+            me = wv == 0 ? QS : (wv == 2 ? 0 : T);
+            if (wv == 2) { hw = 1; hb = 0; he = QS; }
+            helped = wv == 1;
+        } else if (Kb == I) { // rows of qr = 1 need k >= QS: qr 0 [H,2QS), qr 1 [QS,2QS) + [0,H) of (0,qc)
+            mb = qr ? QS : H;
+            if (qr) { hw = qc; hb = 0; he = H; }
+            helped = qr == 0;
+        } else { // columns of qc = 0 need k < QS: qc 1 [0,2QS-H), qc 0 [0,QS) + [2QS-H,2QS) of (qr,1)
+            me = qc ? T - H : QS;
+            if (!qc) { hw = 2 * qr + 1; hb = T - H; he = T; }
+            helped = qc == 1;
+        }
+    }
+    float acc2[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc2[i][j] = NEG_INF;
+    int hr0 = 0, hc0 = 0;
+    if (hw >= 0) lane_block<T>(hr0, hc0, hw * 64 + (int)(threadIdx.x % 64));
     for (int s = it.step_begin; s < it.step_end; ++s) {
         const int K = Kb + s % nK;
         __syncthreads();
@@ -110,9 +145,28 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
 #pragma unroll
                 for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], s1a + Bs[(r0 + i - 1) * LD + c0 + j]);
         }
-        int kb, ke;
-        quad_k_range<T>(K == I, K == J, false, kb, ke);
-        product<T>(acc, As, Bs, r0, c0, kb, ke);
+        int kb = mb, ke = me;
+        if (!balance) quad_k_range<T>(K == I, K == J, false, kb, ke);
+        product<T, H>(acc, As, Bs, r0, c0, kb, ke);
+        if (hw >= 0) product<T, H>(acc2, As, Bs, hr0, hc0, hb, he);
+    }
+    if (balance) { // hand the slices over through the (now free) operand LDS
+        constexpr int RG = R * R * 64;
+        const int l = threadIdx.x % 64;
+        __syncthreads();
+        if (hw >= 0) {
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) smem[hw * RG + i * R * 64 + l * R + j] = acc2[i][j];
+        }
+        __syncthreads();
+        if (helped) {
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], smem[wv * RG + i * R * 64 + l * R + j]);
+        }
     }
 
     if (it.flags & WI_EPILOGUE) {
@@ -825,7 +879,7 @@ __global__ __launch_bounds__(kThreads) void k_tasks(const PairDesc *__restrict__
     case TASK_S1: s1_body(P, arena, codes, w); break;
     case TASK_S2EXT: ext_body<T, true>(P, it, arena, smem); break;
     case TASK_S2INTRA: intra_body<T, true>(P, 0, 0, it.I, it.J, arena, codes, w, smem); break;
-    case TASK_OUTER: outer_body<T>(P, it, arena, codes, w, smem); break;
+    case TASK_OUTER: outer_body<T, false>(P, it, arena, codes, w, smem); break;
     case TASK_EXT: ext_body<T, false>(P, it, arena, smem); break;
     case TASK_INTRA: intra_body<T, false>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem); break;
     }
