@@ -963,6 +963,16 @@ __global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restri
     FetchB<T> fb;
     fetch_A<T, T>(fa, Am, nb, IJ.x, IJ.x, 0);
     fetch_B<T, T>(fb, Bm, nb, IJ.x, IJ.y, 0);
+    // every tile's K range has both diagonal steps: balanced k ranges (balanced_k_range)
+    constexpr int H = Tc<T>::QS / 2;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), hw = balance_target(wv);
+    float acc2[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc2[i][j] = NEG_INF;
+    int hr0 = 0, hc0 = 0;
+    if (hw >= 0) lane_block<T>(hr0, hc0, hw * 64 + (int)(threadIdx.x % 64));
     for (int K = IJ.x; K <= IJ.y; ++K) {
         __syncthreads();
         commit_A<T>(As, fa, 0, false);
@@ -972,10 +982,12 @@ __global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restri
             fetch_A<T, T>(fa, Am, nb, IJ.x, K + 1, 0);
             fetch_B<T, T>(fb, Bm, nb, K + 1, IJ.y, 0);
         }
-        int kb, ke;
-        quad_k_range<T>(K == IJ.x, K == IJ.y, false, kb, ke);
-        product<T>(acc, As, Bs, r0, c0, kb, ke);
+        int kb, ke, hb, he;
+        balanced_k_range<T>(K == IJ.x, K == IJ.y, wv, kb, ke, hb, he);
+        product<T, H>(acc, As, Bs, r0, c0, kb, ke);
+        if (hw >= 0) product<T, H>(acc2, As, Bs, hr0, hc0, hb, he);
     }
+    balance_merge<T>(acc, acc2, As, wv);
     store_acc<T, T>(tile_at(Cm, IJ.x, IJ.y, nb, T), acc, r0, c0, false);
 }
 

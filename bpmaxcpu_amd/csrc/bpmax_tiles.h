@@ -68,6 +68,57 @@ __device__ __forceinline__ void quad_k_range(bool adiag, bool bdiag, bool lastco
     ke = bdiag ? ((w & 1) + 1) * QS : LT;
 }
 
+// Balanced form of quad_k_range for the product steps of a 256-lane workgroup.  On a diagonal
+// step the quadrants' ranges differ (K = I: waves 0,1 2 QS, waves 2,3 QS; K = J: waves 1,3
+// 2 QS, waves 0,2 QS; both: QS, 2 QS, 0, QS), and the step's barrier makes every wave wait
+// for the longest.  Here a lighter wave also runs a slice [hb, he) of a heavier wave's range
+// on that wave's cells: wave 2 helps wave 1, wave 3 wave 0, wave 0 wave 3 (a fixed target
+// per wave, so one extra accumulator serves all steps of an item), and no wave runs more than
+// 1.5 QS per step (QS when K = I = J).  Own range [kb, ke); ranges are multiples of QS/2.
+__device__ __forceinline__ int balance_target(int w) { return w == 0 ? 3 : (w == 2 ? 1 : (w == 3 ? 0 : -1)); }
+template <int LT>
+__device__ __forceinline__ void balanced_k_range(bool adiag, bool bdiag, int w, int &kb, int &ke, int &hb, int &he)
+{
+    constexpr int QS = Tc<LT>::QS, H = QS / 2;
+    kb = 0, ke = LT, hb = 0, he = 0;
+    if (adiag && bdiag) {
+        kb = w == 0 || w == 2 ? 0 : QS;
+        ke = w == 0 ? QS : (w == 2 ? 0 : LT);
+        if (w == 2) he = QS;
+    } else if (adiag) {
+        kb = w < 2 ? H : QS;
+        if (w >= 2) he = H;
+    } else if (bdiag) {
+        ke = w == 0 || w == 2 ? QS : LT - H;
+        if (w == 0 || w == 2) hb = LT - H, he = LT;
+    }
+}
+
+// End of a balanced product: each helper wave hands its slice (acc2, on the cells of wave
+// balance_target(w)) to that wave through LDS (R*R*64 floats per target wave, 4 regions), and
+// the target max-merges it.  `x` must be free LDS; every lane of the workgroup calls this.
+template <int LT>
+__device__ __forceinline__ void balance_merge(float (&acc)[Tc<LT>::R][Tc<LT>::R],
+                                              const float (&acc2)[Tc<LT>::R][Tc<LT>::R], float *x, int w)
+{
+    constexpr int R = Tc<LT>::R, RG = R * R * 64;
+    const int l = threadIdx.x % 64, hw = balance_target(w);
+    __syncthreads();
+    if (hw >= 0) {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j) x[hw * RG + i * R * 64 + l * R + j] = acc2[i][j];
+    }
+    __syncthreads();
+    if (w != 2) { // waves 0, 1 and 3 are targets
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], x[w * RG + i * R * 64 + l * R + j]);
+    }
+}
+
 // Storage tile (I,J) (runtime T), for the layout conversions.
 __device__ __forceinline__ const float *tile_at(const float *tri, int I, int J, int nb, int T)
 {
