@@ -241,7 +241,9 @@ typedef struct {
                         workgroups (one cell per lane: latency), larger ones 256-lane
                         workgroups (one 2x2 block of cells per lane, one step per block
                         diagonal; 512-lane, two cells per lane, with
-                        BPMAX_INTRA_BLOCKED=0)                                            */
+                        BPMAX_INTRA_BLOCKED=0).  With closure_ahead set, F's in-tile
+                        launches are the merged 256-lane ones and this threshold only
+                        selects the form of S2's                                          */
     int outer_kgroup; /* K values (storage tiles of the inner sum) per R0 work item; 0:
                         all of an output tile's.  1 keeps the items of one triangle in
                         step, so each source tile is read from HBM once per k1 and then
