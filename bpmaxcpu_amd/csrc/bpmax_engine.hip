@@ -963,7 +963,8 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
         return rc;
     HIPCHK(ctx, hipEventRecord(ev_fill, sO));
     HIPCHK(ctx, hipStreamWaitEvent(sI, ev_fill, 0));
-    if ((rc = timed(ctx, sI, BPMAX_K_S1, 0, [&] { return launch_s1(sI, dP, npairs, arena, dcodes, dw); })) !=
+    // S1 is read by the R0 launches only: built on their stream, beside the S2 chain
+    if ((rc = timed(ctx, sO, BPMAX_K_S1, 0, [&] { return launch_s1(sO, dP, npairs, arena, dcodes, dw); })) !=
         BPMAX_OK)
         return rc;
     for (const Launch &L : S.launches)
