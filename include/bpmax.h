@@ -72,8 +72,11 @@ const char *bpmax_last_error(const bpmax_ctx *ctx); /* human-readable, never NUL
 /* Solve one pair.
  *   seq1[M], seq2[N]: codes 0..4, seq2 in its natural 5'->3' order (not reversed).
  *   intra, inter: 5x5 weight tables; NULL selects the reference table.  Weights must be
- *     finite, >= 0 and exact multiples of 2^-8 with every reachable score below 2^22, so
- *     max-plus evaluation is exact in any order (BPMAX_EWEIGHTS otherwise).
+ *     finite, >= 0 and all multiples of 2^-p for some p <= 12; with p the smallest such
+ *     exponent and wmax the largest weight, wmax * ((M+N)/2 + 1) * 2^p must stay below
+ *     2^24 (a structure has at most (M+N)/2 pairs, so every partial score is an integer
+ *     multiple of 2^-p below 2^24 * 2^-p, exact in FP32).  Max-plus evaluation is then
+ *     exact in any order; BPMAX_EWEIGHTS otherwise.
  *   F_out: host buffer for `layout`, or NULL with BPMAX_LAYOUT_NONE.
  *   score_out: receives F(0,M-1,0,N-1).
  * M >= 1, N >= 1. */

@@ -10,8 +10,16 @@ stored in each fixture, so nothing depends on a PRNG matching across languages.
 Fixtures:
   full_<M>x<N>_<table>.npz   every defined cell (small shapes)
   hash_<M>x<N>_<table>.json  score, SHA-256 of the packed float32 table, strided samples
+  set_<name>_<table>.json    a list of pairs drawn from one workload's distribution, each
+                             with its score, SHA-256 and samples:
+                               batch4  16 pairs, M~U[16,48], N~U[100,400], per-sequence
+                                       GC~U[0.3,0.7] (the batched workload);
+                               serve5  10 pairs, M log-uniform [8,64], N log-uniform
+                                       [64,1024], GC 0.5 (the serving workload), with its
+                                       corner shapes forced in (8x1024, 12x800, 40x300,
+                                       48x700, 64x1024, 56x96, 8x64)
 
-Usage:  python tests/golden/make_golden.py [--verify-oracle]
+Usage:  python tests/golden/make_golden.py [--verify-oracle] [--sets batch4,serve5] [--no-single]
   --verify-oracle also runs the C restatement (oracle/bpmax_oracle.c) on every fixture and
   requires bit-identical tables (the 64x512 case takes about a minute on 8 cores).
 """
@@ -66,13 +74,76 @@ def samples(F, M, N, n=64, seed=99):
     return out
 
 
+def gc_seq(rng, n, gc):
+    """i.i.d. bases with P(G)=P(C)=gc/2, P(A)=P(U)=(1-gc)/2 (codes A0 C1 G2 U3)."""
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return rng.choice(4, size=n, p=p).astype(np.int32)
+
+
+def set_pairs(name):
+    """The pairs of fixture set `name` (sequences drawn here, stored in the fixture)."""
+    if name == "batch4":
+        rng = np.random.default_rng(404)
+        out = []
+        for _ in range(16):
+            M = int(rng.integers(16, 49))
+            N = int(rng.integers(100, 401))
+            out.append((gc_seq(rng, M, rng.uniform(0.3, 0.7)), gc_seq(rng, N, rng.uniform(0.3, 0.7))))
+        return out
+    if name == "serve5":
+        rng = np.random.default_rng(505)
+        shapes = [(8, 1024), (12, 800), (40, 300), (48, 700), (64, 1024), (56, 96), (8, 64)]
+        for _ in range(3):
+            shapes.append((int(round(np.exp(rng.uniform(np.log(8), np.log(64))))),
+                           int(round(np.exp(rng.uniform(np.log(64), np.log(1024)))))))
+        return [(gc_seq(rng, M, 0.5), gc_seq(rng, N, 0.5)) for M, N in shapes]
+    raise ValueError(name)
+
+
+# per set: the tables written, and pairs left out of a table (reference CPU time)
+SET_TABLES = {"batch4": ("ref", "321"), "serve5": ("ref", "321")}
+SET_SKIP = {("serve5", "321"): {(64, 1024)}}
+
+
+def write_set(name, tables, verify):
+    for tname in tables:
+        if tname not in SET_TABLES[name]:
+            continue
+        intra, inter = TABLES[tname]()
+        entries = []
+        for k, (s1, s2) in enumerate(set_pairs(name)):
+            M, N = len(s1), len(s2)
+            if (M, N) in SET_SKIP.get((name, tname), ()):
+                continue
+            F, score, secs, ntile = O.reference_run(s1, s2, intra, inter)
+            if verify and M * N <= 48 * 400:
+                Fo, so = O.bpmax(s1, s2, intra, inter)
+                assert np.array_equal(Fo.view(np.uint32), F.view(np.uint32)), (name, k, tname)
+            entries.append(dict(index=k, M=M, N=N, seq1=digits(s1), seq2=digits(s2), score=score,
+                                n_tile=ntile, sha256=sha(F), samples=samples(F, M, N, n=16)))
+            print(f"set {name} {tname} pair {k} {M}x{N}: score={score} ref_time={secs:.2f}s", flush=True)
+            del F
+        path = os.path.join(HERE, f"set_{name}_{tname}.json")
+        with open(path, "w") as f:
+            json.dump(dict(set=name, table=tname, intra=intra.reshape(-1).tolist(),
+                           inter=inter.reshape(-1).tolist(), pairs=entries,
+                           source="reference bpmax() (src/bpmax.c:129) via oracle/ref_harness.cpp"), f, indent=1)
+        print(f"{os.path.basename(path)}: {len(entries)} pairs", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--verify-oracle", action="store_true")
     ap.add_argument("--tables", default=",".join(TABLES), help="comma-separated subset of " + ",".join(TABLES))
+    ap.add_argument("--sets", default="", help="comma-separated fixture sets to write (batch4, serve5)")
+    ap.add_argument("--no-single", action="store_true", help="skip the full_* and hash_* fixtures")
     args = ap.parse_args()
     if not O.have_reference():
         sys.exit("oracle/_ref/bpmax_ref_run missing: run `make -C oracle ref` first")
+    for name in filter(None, args.sets.split(",")):
+        write_set(name, args.tables.split(","), args.verify_oracle)
+    if args.no_single:
+        return
     for (M, N, seed) in FULL + HASHED:
         s1, s2 = seqs(M, N, seed)
         for tname in args.tables.split(","):
