@@ -31,7 +31,8 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-FP32_VALU_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector peak (packed FMA counted as 4 flops/lane)
+FP32_VALU_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector peak: 256 CUs x 2.4 GHz x 128 lanes x 2 (FMA)
+VALU_ISSUE_PEAK = 2 * 256 * 2.4e9  # wave64 FP32 VALU instructions per second (2 per clock per CU)
 
 
 def log(msg):
@@ -150,6 +151,19 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
                 "kernel": "k_outer<64>: R0 double max-plus + init/P1/R3/R4 (2 flops per cell update)",
                 "launches": o["launches"], "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops_per_launch}
+    vpath = os.path.join(REPO, "profiles", "k_outer_64x512_valu.json")
+    if (M, N) == (64, 512) and os.path.exists(vpath) and o["launches"] and avg_s > 0:
+        # rocprofv3 SQ_INSTS_VALU of one solve (calibrated to count wave instructions exactly)
+        # spread over this run's launches per solve, against the issue peak of two wave64
+        # FP32 instructions per clock per CU
+        with open(vpath) as fh:
+            per_solve = json.load(fh)["sq_insts_valu_per_solve"]
+        per_launch = per_solve * steps / o["launches"]
+        rate = per_launch / avg_s
+        roofline["valu_issue"] = {"wave_instr_per_launch": per_launch, "achieved": rate,
+                                  "peak": VALU_ISSUE_PEAK, "unit": "wave VALU instructions/s",
+                                  "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                                  "source": "profiles/k_outer_64x512_valu.md"}
 
     # one-stream schedule, for the phase breakdown without overlap
     eng.set_tuning(M, N, 1, {"schedule": "launch", "outer_split": 1, "outer_items": tuned["outer_items"],
@@ -337,7 +351,26 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (reads MIB, writes MIB): the "
                          "reference point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
+    ap.add_argument("--calibrate-valu", action="store_true",
+                    help="only run the VALU counter-calibration kernel (libbpmax_probe.so): two launches "
+                         "of 2048 workgroups x 4 waves with 32 x 1000 and 32 x 4000 VALU instructions per "
+                         "wave, the reference points that scale rocprofv3 SQ_INSTS_VALU to wave instructions")
     args = ap.parse_args()
+    if args.calibrate_valu:
+        import ctypes
+        P = ctypes.CDLL(os.path.join(REPO, "bpmaxcpu_amd", "libbpmax_probe.so"))
+        P.bpmax_probe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        out = []
+        for iters in (1000, 4000):
+            ms = ctypes.c_double()
+            rc = P.bpmax_probe_valu(2048, iters, ctypes.byref(ms))
+            if rc != 0:
+                raise SystemExit(f"probe failed: {rc}")
+            waves = 2048 * 4
+            out.append({"iters": iters, "waves": waves, "valu_per_wave_loop": 32 * iters, "ms": round(ms.value, 4),
+                        "wave_valu_per_clock_per_cu": round(waves * 32 * iters / (ms.value * 1e-3) / 2.4e9 / 256, 3)})
+        print(json.dumps({"valu_probe": out}), flush=True)
+        return
     if args.calibrate_copy:
         import torch
         n = (args.calibrate_copy << 20) // 4

@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
         "bpmax_engine_score_table": (None, [fp, fp]),
         "bpmax_engine_solve": (c_int, [c_void_p, ip, c_int, ip, c_int, fp, fp, fp, c_int, fp]),
         "bpmax_engine_solve_batch": (c_int, [c_void_p, c_int, ipp, ip, ipp, ip, fp, fp, fp]),
+        "bpmax_engine_fetch": (c_int, [c_void_p, c_int, c_int, fp]),
         "bpmax_layout_bytes": (c_size_t, [c_int, c_int, c_int]),
         "bpmax_engine_pair_device_bytes": (c_size_t, [c_int, c_int]),
         "bpmax_tri_tiled_floats": (c_size_t, [c_int]),
@@ -265,6 +266,15 @@ class Engine:
                                              None if te is None else _fp(te), _fp(scores))
         self._check(rc)
         return scores
+
+    def fetch(self, pair: int, M: int, N: int, layout: int = LAYOUT_PACKED) -> np.ndarray:
+        """F of pair `pair` of the last solve / solve_batch (bpmax_engine_fetch)."""
+        if layout == LAYOUT_PACKED:
+            out = np.empty((M * (M + 1) // 2) * (N * (N + 1) // 2), np.float32)
+        else:
+            out = np.empty((M, M, N, N), np.float32)
+        self._check(self.L.bpmax_engine_fetch(self.h, pair, layout, _fp(out)))
+        return out
 
     # ------------------------------------------------------- isolated R0 kernel
     def tri_tiled_floats(self, n: int) -> int:

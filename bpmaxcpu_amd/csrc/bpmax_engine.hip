@@ -1270,11 +1270,23 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
     return BPMAX_OK;
 }
 
+static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms, const int *const *seq2s,
+                     const int *Ns, const float *intra_in, const float *inter_in, bool sync);
+
 // Lay out pairs in the arena, upload codes/descriptors/weights/items, run the schedule.
 // On success the F tables of all pairs are resident in ctx->arena and ctx->pairs_host
-// describes them.
+// describes them (ctx->result_valid).
 int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms, const int *const *seq2s,
                   const int *Ns, const float *intra_in, const float *inter_in, bool sync)
+{
+    ctx->result_valid = false;
+    const int rc = run_pairs(ctx, npairs, seq1s, Ms, seq2s, Ns, intra_in, inter_in, sync);
+    ctx->result_valid = rc == BPMAX_OK;
+    return rc;
+}
+
+static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms, const int *const *seq2s,
+                     const int *Ns, const float *intra_in, const float *inter_in, bool sync)
 {
     if (npairs <= 0) return BPMAX_EINVAL;
     float wt[50];
@@ -1532,6 +1544,20 @@ extern "C" int bpmax_engine_solve(bpmax_ctx *ctx, const int *seq1, int M, const 
     if (score_out) *score_out = score;
     if (layout == BPMAX_LAYOUT_NONE) return BPMAX_OK;
     return bpx_copy_out(ctx, bpx_arena(ctx), ctx->pairs_host[0], layout, F_out);
+}
+
+extern "C" int bpmax_engine_fetch(bpmax_ctx *ctx, int pair, bpmax_layout layout, float *F_out)
+{
+    if (!ctx) return BPMAX_EINVAL;
+    if (!F_out || (layout != BPMAX_LAYOUT_PACKED_TRI_TRI && layout != BPMAX_LAYOUT_DENSE_4D)) {
+        ctx->err = "fetch needs F_out and a packed or dense layout";
+        return BPMAX_EINVAL;
+    }
+    if (!ctx->result_valid || ctx->arena_ext || pair < 0 || pair >= (int)ctx->pairs_host.size()) {
+        ctx->err = "no resident result for pair " + std::to_string(pair);
+        return BPMAX_EINVAL;
+    }
+    return bpx_copy_out(ctx, bpx_arena(ctx), ctx->pairs_host[pair], layout, F_out);
 }
 
 int bpx_copy_out(bpmax_ctx *ctx, const float *arena, const PairDesc &P, bpmax_layout layout, float *F_out)

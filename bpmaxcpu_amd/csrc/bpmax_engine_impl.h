@@ -81,6 +81,7 @@ struct bpmax_ctx {
     int schedule = BPMAX_SCHED_PIPE;
     int task_chunk = 16;
     std::vector<bpx::PairDesc> pairs_host;
+    bool result_valid = false;           // pairs_host's F tables are resident in the arena
     bpx::Schedule sched;
     std::vector<long long> sched_key;   // (schedule, chunk, shapes) the schedule was built for
     void *items_on_device = nullptr;    // meta buffer that already holds sched.items

@@ -99,6 +99,12 @@ size_t bpmax_engine_pair_device_bytes(int M, int N);
 int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s,
                              const int *Ms, const int *const *seq2s, const int *Ns,
                              const float *intra, const float *inter, float *scores);
+/* Copy the F table of pair `pair` (0-based, in call order) of the context's last
+ * bpmax_engine_solve or bpmax_engine_solve_batch, still resident on the device, to F_out in
+ * `layout` (PACKED_TRI_TRI or DENSE_4D).  BPMAX_EINVAL when the last solve failed, another
+ * call has since reused the arena, or `pair` is out of range.  No counterpart in the
+ * reference (its bpmax() writes F into the caller's table directly). */
+int bpmax_engine_fetch(bpmax_ctx *ctx, int pair, bpmax_layout layout, float *F_out);
 
 /* ---- serving loop ---------------------------------------------------------------------- */
 /* Continuous batching over one device arena of `memory_budget` bytes.  Submitted pairs

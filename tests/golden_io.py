@@ -40,3 +40,20 @@ def sha256_f32(F):
 
 def fid(path):
     return os.path.basename(path).rsplit(".", 1)[0]
+
+
+def load_set(name):
+    """A fixture set (set_<workload>_<table>.json): the table and a list of pairs, each
+    with seq1/seq2 as code arrays, score, sha256 and samples."""
+    with open(os.path.join(GOLDEN, name)) as f:
+        s = json.load(f)
+    s["intra"] = np.array(s["intra"], np.float32).reshape(5, 5)
+    s["inter"] = np.array(s["inter"], np.float32).reshape(5, 5)
+    for p in s["pairs"]:
+        p["seq1"] = np.array([int(c) for c in p["seq1"]], np.int32)
+        p["seq2"] = np.array([int(c) for c in p["seq2"]], np.int32)
+    return s
+
+
+def set_fixtures(workload):
+    return sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, f"set_{workload}_*.json")))
