@@ -153,3 +153,31 @@ def test_invariants_random_pair():
                 k2 = int(rng.integers(i2, j2))
                 assert v >= F[idx(i1, k1, i2, k2)] + F[idx(k1 + 1, j1, k2 + 1, j2)]
     assert score == F[idx(0, M - 1, 0, N - 1)]
+
+
+def _small_set_pairs(workload, max_cells):
+    out = []
+    for name in G.set_fixtures(workload):
+        fx = G.load_set(name)
+        for p in fx["pairs"]:
+            if O.tri(p["M"]) * O.tri(p["N"]) <= max_cells:
+                out.append((name, fx["intra"], fx["inter"], p))
+    return out
+
+
+@pytest.mark.parametrize("workload,max_cells", [("batch4", 4.0e6), ("serve5", 1.0e7), ("serve5-1024", 0)])
+def test_oracle_on_workload_fixture_sets(workload, max_cells):
+    """The oracle against the reference's tables for the pairs of the batched and serving
+    fixture sets that it finishes in a few seconds, and for the 8x1024 pair (reference
+    table): N = 1024, the largest inner length of the serving workload."""
+    if workload == "serve5-1024":
+        cases = [c for c in _small_set_pairs("serve5", 2.0e7) if (c[3]["M"], c[3]["N"]) == (8, 1024)
+                 and c[0].endswith("_ref.json")]
+        assert len(cases) == 1
+    else:
+        cases = _small_set_pairs(workload, max_cells)
+        assert len(cases) >= 2
+    for name, intra, inter, p in cases:
+        F, score = O.bpmax(p["seq1"], p["seq2"], intra, inter)
+        assert score == p["score"], (name, p["M"], p["N"])
+        assert G.sha256_f32(F) == p["sha256"], (name, p["M"], p["N"])
