@@ -568,20 +568,171 @@ static int pick_chunk(long long total_steps, long long tiles)
     return (int)ch;
 }
 
-// Build the per-phase launch list for a set of pairs (descs already laid out), one group
-// per storage tile size T:
-//   S2 by logical block diagonal (cross-tile launch, then closure launch);
-//   for every outer diagonal d1: R0 launches over ranges of storage block diagonals (by
-//   default one launch per storage diagonal D: all storage tiles (I,I+D) of every triangle
-//   on d1; outer_split > 0 merges them into that many ranges), then the inner closure by
-//   logical block diagonal DL (cross-tile launch, closure launch) -- or, with
-//   closure_ahead, one launch per DL (k_inner_closure_blk): the in-tile items of DL, whose
-//   workgroups first run their tiles' cross-tile steps D-1 and D (the two that read
-//   diagonal DL-1), and cross-tile items with the other steps of the tiles on DL+1 (they
-//   read diagonals below DL only).
-// Every range / (d1, DL) position gets a Launch entry, possibly empty, so the pipelined
-// runner can attach its events to each.  An R0 launch's `diag` is the last storage
-// diagonal of its range.
+// Launch lists are built from two pieces, both over the pairs of one storage-tile group g
+// (pairs sharing T):
+//   emit_s2:   S2 by logical block diagonal (cross-tile launch, then closure launch) for the
+//              pairs with sel[p] set;
+//   emit_tick: one step of the outer wavefront -- every selected pair p advances its outer
+//              diagonal d1s[p] (pairs may sit on different diagonals): R0 launches over
+//              `split` ranges of storage block diagonals (all storage tiles (I,I+Ds) of every
+//              triangle (i1,i1+d1s[p]), Ds in the range), then the inner closure by logical
+//              block diagonal DL (cross-tile launch, closure launch) -- or, with
+//              closure_ahead, one launch per DL (k_inner_closure_blk): the in-tile items of
+//              DL, whose workgroups first run their tiles' cross-tile steps D-1 and D (the two
+//              that read diagonal DL-1), and cross-tile items with the other steps of the
+//              tiles on DL+1 (they read diagonals below DL only).
+// nbmax / nblmax: storage / logical tiles per side the tick's ranges and levels cover (the
+// largest of the group); R0 range r ends at storage diagonal out_end of its last Ds.
+// Every range / DL position gets a Launch entry, possibly empty, so that a runner can
+// attach its events to each.  An R0 launch's `diag` is the last storage diagonal of its
+// range; launches carry the tick index in `d1`.
+static void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g, int T, int nblmax,
+                    Schedule &S)
+{
+    auto emit = [&](int kind, size_t first, int diag) {
+        S.launches.push_back(Launch{kind, T, true, (int)first, (int)(S.items.size() - first), 0, g, -1, diag});
+    };
+    for (int D = 0; D < nblmax; ++D) {
+        if (D > 0) {
+            size_t f = S.items.size();
+            long long tot = 0;
+            for (size_t p = 0; p < P.size(); ++p)
+                if (sel[p] && P[p].T == T && P[p].nbl > D) tot += (long long)(P[p].nbl - D) * D;
+            const int ch = pick_chunk(tot, 0);
+            for (size_t p = 0; p < P.size(); ++p) {
+                if (!sel[p] || P[p].T != T || P[p].nbl <= D) continue;
+                for (int I = 0; I + D < P[p].nbl; ++I)
+                    chunked(S.items, WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0}, D, ch, false);
+            }
+            emit(BPMAX_K_INNER_EXT, f, D);
+        }
+        size_t f = S.items.size();
+        for (size_t p = 0; p < P.size(); ++p) {
+            if (!sel[p] || P[p].T != T || P[p].nbl <= D) continue;
+            for (int I = 0; I + D < P[p].nbl; ++I) S.items.push_back(WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0});
+        }
+        emit(BPMAX_K_INNER_INTRA, f, D);
+    }
+}
+
+void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nbmax, int nblmax,
+               int tick, Schedule &S, std::vector<int> &out_end, int outer_split, int outer_items,
+               int outer_kgroup, bool closure_ahead)
+{
+    auto active = [&](size_t p) { return P[p].T == T && d1s[p] >= 0 && d1s[p] < P[p].M; };
+    auto emit = [&](int kind, size_t first, double upd, int diag) {
+        S.launches.push_back(Launch{kind, T, false, (int)first, (int)(S.items.size() - first), upd, g, tick, diag});
+    };
+    // R0 launches: storage diagonals in `split` contiguous ranges
+    const int split = outer_split <= 0 ? nbmax : std::min(outer_split, nbmax);
+    std::vector<int> range_end(split);
+    out_end.assign(nbmax, 0);
+    for (int r = 0, Ds = 0; r < split; ++r) {
+        range_end[r] = (int)((long long)nbmax * (r + 1) / split) - 1;
+        for (; Ds <= range_end[r]; ++Ds) out_end[Ds] = range_end[r];
+    }
+    for (int r = 0; r < split; ++r) {
+        const int Da = r == 0 ? 0 : range_end[r - 1] + 1, Db = range_end[r];
+        size_t f = S.items.size();
+        // K values per item: a tile's K range split into groups of kg, so that every item of
+        // a (triangle, k1 chunk) takes about one product step per k1 and the items that run
+        // together stay aligned on k1 (each source tile read from L2 by all of its consumers
+        // at about the same time)
+        auto kg_of = [&](int d1) { return d1 == 0 || nbmax > 255 ? 0 : outer_kgroup; };
+        auto groups = [&](int kg, int nK) { return kg > 0 ? (nK + kg - 1) / kg : 1; };
+        long long steps = 0;
+        double upd = 0;
+        for (size_t p = 0; p < P.size(); ++p) {
+            if (!active(p)) continue;
+            const int d1 = d1s[p], kg = kg_of(d1);
+            long long tiles = 0;
+            for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
+                tiles += (long long)(P[p].M - d1) * (P[p].nb - Ds) * groups(kg, Ds + 1);
+            steps += (long long)d1 * tiles;
+            if (r == 0) {
+                const double n = P[p].N;
+                const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
+                upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
+            }
+        }
+        // Items cover whole k1 values (ck of them, every K of each), ordered triangle,
+        // k1-chunk, tile: workgroups running together then read the same source tiles (row I
+        // of F(i1,k1), column J of F(k1+1,j1)) and share them in L2.
+        const long long want = std::max(64, outer_items);
+        const int ck = (int)std::max<long long>(1, (steps + want - 1) / want);
+        for (size_t p = 0; p < P.size(); ++p) {
+            if (!active(p)) continue;
+            const int d1 = d1s[p], kg = kg_of(d1);
+            const int nck = d1 == 0 ? 1 : (d1 + ck - 1) / ck;
+            for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                for (int c = 0; c < nck; ++c)
+                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
+                        for (int I = 0; I + Ds < P[p].nb; ++I)
+                            for (int gi = 0, ng = groups(kg, Ds + 1); gi < ng; ++gi) {
+                                const int koff = kg > 0 ? gi * kg : 0;
+                                const int nK = kg > 0 ? std::min(kg, Ds + 1 - koff) : Ds + 1;
+                                const int sb = c * ck * nK, se = std::min(d1, (c + 1) * ck) * nK;
+                                int fl = (nck > 1 || ng > 1 ? WI_ATOMIC : 0) | (c == 0 && gi == 0 ? WI_EPILOGUE : 0);
+                                if (kg > 0) fl |= (koff << WI_KOFF_SHIFT) | (nK << WI_KCNT_SHIFT);
+                                S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + Ds, sb, se, fl});
+                            }
+        }
+        split_tail(S.items, f);
+        emit(BPMAX_K_OUTER, f, upd, Db);
+    }
+    // inner closure by logical block diagonal
+    for (int D = 0; D < nblmax; ++D) {
+        if (D > 0 && !closure_ahead) {
+            size_t f = S.items.size();
+            long long tot = 0;
+            for (size_t p = 0; p < P.size(); ++p)
+                if (active(p) && P[p].nbl > D) tot += (long long)(P[p].M - d1s[p]) * (P[p].nbl - D) * 2 * D;
+            const int ch = pick_chunk(tot, 0);
+            for (size_t p = 0; p < P.size(); ++p) {
+                if (!active(p) || P[p].nbl <= D) continue;
+                const int d1 = d1s[p];
+                for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
+                    for (int I = 0; I + D < P[p].nbl; ++I)
+                        chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, 0}, 2 * D, ch, false);
+            }
+            emit(BPMAX_K_INNER_EXT, f, 0, D);
+        }
+        size_t f = S.items.size();
+        const int tin = closure_ahead ? TASK_INTRA << WI_TYPE_SHIFT : 0;
+        // closure ahead: steps [0, D) and [D+2, 2D+2) of the tiles on D+1, in items longer
+        // than the cross-tile launches' (these share launches with the in-tile work; 64x512,
+        // 24x512 and 48x384 ran faster with 8..32 steps than with 4 or with the cross-tile
+        // launches' chunks)
+        const int D1 = D + 1;
+        long long tot = 0;
+        for (size_t p = 0; p < P.size() && closure_ahead; ++p)
+            if (active(p) && P[p].nbl > D1) tot += (long long)(P[p].M - d1s[p]) * (P[p].nbl - D1) * 2 * D;
+        const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
+        const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
+        // by triangle, each triangle's in-tile items then its cross-tile items, so that every
+        // XCD's contiguous share of the launch mixes the two kinds
+        for (size_t p = 0; p < P.size(); ++p) {
+            if (!active(p) || P[p].nbl <= D) continue;
+            const int d1 = d1s[p];
+            for (int i1 = 0; i1 + d1 < P[p].M; ++i1) {
+                for (int I = 0; I + D < P[p].nbl; ++I)
+                    S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, tin});
+                if (!closure_ahead || D == 0) continue;
+                for (int I = 0; I + D1 < P[p].nbl; ++I) {
+                    const WorkItem base{(int)p, i1, i1 + d1, I, I + D1, 0, 0, tex};
+                    chunked_range(S.items, base, 0, D, ch);
+                    chunked_range(S.items, base, D + 2, 2 * D1, ch);
+                }
+            }
+        }
+        emit(BPMAX_K_INNER_INTRA, f, 0, D);
+        S.launches.back().ahead = closure_ahead;
+    }
+}
+
+// The launch list of a whole solve of a set of pairs (descs already laid out), one group per
+// storage tile size T: S2 of every pair, then one tick per outer diagonal d1 with every
+// pair on d1 (pairs with M <= d1 are done).
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
                     bool closure_ahead)
 {
@@ -601,137 +752,15 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
         G.maxnb = std::max(G.maxnb, p.nb);
         G.maxnbl = std::max(G.maxnbl, p.nbl);
     }
-    auto emit = [&](int kind, int g, bool ss, size_t first, double upd, int d1, int diag) {
-        const int n = (int)(S.items.size() - first);
-        S.launches.push_back(Launch{kind, S.groups[g].T, ss, (int)first, n, upd, g, d1, diag});
-    };
+    const std::vector<char> all(P.size(), 1);
+    std::vector<int> d1s(P.size());
     for (int g = 0; g < (int)S.groups.size(); ++g) {
-        const GroupInfo &G = S.groups[g];
-        const int T = G.T;
-        // S2, by logical block diagonal
-        for (int D = 0; D < G.maxnbl; ++D) {
-            if (D > 0) {
-                size_t f = S.items.size();
-                long long tot = 0;
-                for (size_t p = 0; p < P.size(); ++p)
-                    if (P[p].T == T && P[p].nbl > D) tot += (long long)(P[p].nbl - D) * D;
-                const int ch = pick_chunk(tot, 0);
-                for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].nbl <= D) continue;
-                    for (int I = 0; I + D < P[p].nbl; ++I)
-                        chunked(S.items, WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0}, D, ch, false);
-                }
-                emit(BPMAX_K_INNER_EXT, g, true, f, 0, -1, D);
-            }
-            size_t f = S.items.size();
-            for (size_t p = 0; p < P.size(); ++p) {
-                if (P[p].T != T || P[p].nbl <= D) continue;
-                for (int I = 0; I + D < P[p].nbl; ++I) S.items.push_back(WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0});
-            }
-            emit(BPMAX_K_INNER_INTRA, g, true, f, 0, -1, D);
-        }
-        // R0 launches per outer diagonal: storage diagonals in `split` contiguous ranges
-        const int split = outer_split <= 0 ? G.maxnb : std::min(outer_split, G.maxnb);
-        std::vector<int> range_end(split);
-        S.groups[g].out_end.assign(G.maxnb, 0);
-        for (int r = 0, Ds = 0; r < split; ++r) {
-            range_end[r] = (int)((long long)G.maxnb * (r + 1) / split) - 1;
-            for (; Ds <= range_end[r]; ++Ds) S.groups[g].out_end[Ds] = range_end[r];
-        }
-        // F table, by outer diagonal
+        GroupInfo &G = S.groups[g];
+        emit_s2(P, all, g, G.T, G.maxnbl, S);
         for (int d1 = 0; d1 < G.maxM; ++d1) {
-            for (int r = 0; r < split; ++r) {
-                const int Da = r == 0 ? 0 : range_end[r - 1] + 1, Db = range_end[r];
-                size_t f = S.items.size();
-                // K values per item: a tile's K range split into groups of kg, so that every
-                // item of a (triangle, k1 chunk) takes about one product step per k1 and the
-                // items that run together stay aligned on k1 (each source tile read from L2
-                // by all of its consumers at about the same time)
-                const int kg = d1 == 0 || G.maxnb > 255 ? 0 : outer_kgroup;
-                auto groups = [&](int nK) { return kg > 0 ? (nK + kg - 1) / kg : 1; };
-                long long tiles = 0;
-                double upd = 0;
-                for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].M <= d1) continue;
-                    for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
-                        tiles += (long long)(P[p].M - d1) * (P[p].nb - Ds) * groups(Ds + 1);
-                    if (r == 0) {
-                        const double n = P[p].N;
-                        const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
-                        upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
-                    }
-                }
-                // Items cover whole k1 values (ck of them, every K of each), ordered triangle,
-                // k1-chunk, tile: workgroups running together then read the same source tiles
-                // (row I of F(i1,k1), column J of F(k1+1,j1)) and share them in L2.
-                const long long want = std::max(64, outer_items);
-                const int ck = d1 == 0 ? 1 : (int)std::max<long long>(1, (d1 * tiles + want - 1) / want);
-                const int nck = d1 == 0 ? 1 : (d1 + ck - 1) / ck;
-                for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].M <= d1) continue;
-                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                        for (int c = 0; c < nck; ++c)
-                            for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
-                                for (int I = 0; I + Ds < P[p].nb; ++I)
-                                    for (int g = 0, ng = groups(Ds + 1); g < ng; ++g) {
-                                        const int koff = kg > 0 ? g * kg : 0;
-                                        const int nK = kg > 0 ? std::min(kg, Ds + 1 - koff) : Ds + 1;
-                                        const int sb = c * ck * nK, se = std::min(d1, (c + 1) * ck) * nK;
-                                        int fl = (nck > 1 || ng > 1 ? WI_ATOMIC : 0) | (c == 0 && g == 0 ? WI_EPILOGUE : 0);
-                                        if (kg > 0) fl |= (koff << WI_KOFF_SHIFT) | (nK << WI_KCNT_SHIFT);
-                                        S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + Ds, sb, se, fl});
-                                    }
-                }
-                split_tail(S.items, f);
-                emit(BPMAX_K_OUTER, g, false, f, upd, d1, Db);
-            }
-            for (int D = 0; D < G.maxnbl; ++D) {
-                if (D > 0 && !closure_ahead) {
-                    size_t f = S.items.size();
-                    long long tot = 0;
-                    for (size_t p = 0; p < P.size(); ++p)
-                        if (P[p].T == T && P[p].M > d1 && P[p].nbl > D)
-                            tot += (long long)(P[p].M - d1) * (P[p].nbl - D) * 2 * D;
-                    const int ch = pick_chunk(tot, 0);
-                    for (size_t p = 0; p < P.size(); ++p) {
-                        if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D) continue;
-                        for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                            for (int I = 0; I + D < P[p].nbl; ++I)
-                                chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, 0}, 2 * D, ch, false);
-                    }
-                    emit(BPMAX_K_INNER_EXT, g, false, f, 0, d1, D);
-                }
-                size_t f = S.items.size();
-                const int tin = closure_ahead ? TASK_INTRA << WI_TYPE_SHIFT : 0;
-                // closure ahead: steps [0, D) and [D+2, 2D+2) of the tiles on D+1, in items
-                // longer than the cross-tile launches' (these share launches with the in-tile
-                // work; 64x512, 24x512 and 48x384 ran faster with 8..32 steps than with 4 or
-                // with the cross-tile launches' chunks)
-                const int D1 = D + 1;
-                long long tot = 0;
-                for (size_t p = 0; p < P.size() && closure_ahead; ++p)
-                    if (P[p].T == T && P[p].M > d1 && P[p].nbl > D1)
-                        tot += (long long)(P[p].M - d1) * (P[p].nbl - D1) * 2 * D;
-                const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
-                const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
-                // by triangle, each triangle's in-tile items then its cross-tile items, so
-                // that every XCD's contiguous share of the launch mixes the two kinds
-                for (size_t p = 0; p < P.size(); ++p) {
-                    if (P[p].T != T || P[p].M <= d1 || P[p].nbl <= D) continue;
-                    for (int i1 = 0; i1 + d1 < P[p].M; ++i1) {
-                        for (int I = 0; I + D < P[p].nbl; ++I)
-                            S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, tin});
-                        if (!closure_ahead || D == 0) continue;
-                        for (int I = 0; I + D1 < P[p].nbl; ++I) {
-                            const WorkItem base{(int)p, i1, i1 + d1, I, I + D1, 0, 0, tex};
-                            chunked_range(S.items, base, 0, D, ch);
-                            chunked_range(S.items, base, D + 2, 2 * D1, ch);
-                        }
-                    }
-                }
-                emit(BPMAX_K_INNER_INTRA, g, false, f, 0, d1, D);
-                S.launches.back().ahead = closure_ahead;
-            }
+            std::fill(d1s.begin(), d1s.end(), d1);
+            emit_tick(P, d1s, g, G.T, G.maxnb, G.maxnbl, d1, S, G.out_end, outer_split, outer_items, outer_kgroup,
+                      closure_ahead);
         }
     }
 }

@@ -65,6 +65,11 @@ size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
                     bool closure_ahead);
+// One step of the outer wavefront for group g: pair p on outer diagonal d1s[p] (< 0: not in
+// the tick); out_end receives the last storage diagonal of the R0 range holding each Ds.
+void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nbmax, int nblmax,
+               int tick, Schedule &S, std::vector<int> &out_end, int outer_split, int outer_items,
+               int outer_kgroup, bool closure_ahead);
 long long tuning_class(int M, int N, int npairs);
 unsigned long long splitmix64_host(unsigned long long x);
 void build_tasks(const std::vector<PairDesc> &P, int T, int chunk, std::vector<WorkItem> &tasks,
