@@ -61,19 +61,51 @@ def mixed_pairs(npairs=256, seed=11):
     return s1, s2
 
 
-def cpu_baseline(M, N, s1, s2, threads):
-    """Reference bpmax() on the host (kind 'reference'), or the oracle restatement on a
-    bounded sample when the reference binary is absent (kind 'port')."""
+def cpu_info():
+    """CPU model and topology of this host (lscpu), and the CPUs this process may use."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+    except (OSError, subprocess.SubprocessError):
+        out = ""
+    d = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            d[k.strip()] = v.strip()
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": d.get("Model name"), "sockets": d.get("Socket(s)"), "cores_per_socket": d.get("Core(s) per socket"),
+            "threads_per_core": d.get("Thread(s) per core"), "cpus_online": os.cpu_count(), "cpus_usable": usable}
+
+
+def cpu_baseline(M, N, s1, s2, threads, single_M=32):
+    """Reference bpmax() on the host (kind 'reference'): 1 warm-up run, then the median of 3,
+    with OMP_PROC_BIND=close and OMP_PLACES=cores, on `threads` threads for the full pair and
+    on one thread for a single_M x N sample (the first single_M bases of seq1).  The oracle
+    restatement on a bounded sample when the reference binary is absent (kind 'port')."""
     from oracle import oracle as O
     import bpmaxcpu_amd as B
+    info = cpu_info()
     if O.have_reference():
-        log(f"cpu baseline: reference bpmax() {M}x{N} on {threads} threads")
-        _, score, secs, _ = O.reference_run(s1, s2, table=False, threads=threads)
+        env = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
+        log(f"cpu baseline: reference bpmax() {M}x{N} on {threads} threads, 1 warm-up + median of 3")
+        _, score, secs, _ = O.reference_run(s1, s2, table=False, threads=threads, warmup=1, repeat=3, env_extra=env)
         cells = B.cell_updates(M, N)
+        Ms = min(single_M, M)
+        log(f"cpu baseline: reference bpmax() {Ms}x{N} on 1 thread, 1 warm-up + median of 3")
+        _, _, secs1, _ = O.reference_run(s1[:Ms], s2, table=False, threads=1, warmup=1, repeat=3, env_extra=env)
         return {"value": round(cells / secs / 1e9, 3), "unit": "GCUPS", "cores": threads, "kind": "reference",
                 "sample": f"one {M}x{N} pair (the full workload), reference [MPT+RT]:v3 bpmax() built with "
                           f"g++ -O3 -fopenmp -mavx2 -mfma + FTZ, N_tile 48, 3x24 register tile, "
-                          f"OMP_NUM_THREADS={threads}; {secs:.2f} s, score {score}"}
+                          f"OMP_NUM_THREADS={threads}, OMP_PROC_BIND=close, OMP_PLACES=cores; 1 warm-up run, "
+                          f"median of 3: {secs:.2f} s, score {score}",
+                "cpu": info,
+                "single_core": {"value": round(B.cell_updates(Ms, N) / secs1 / 1e9, 3), "unit": "GCUPS", "cores": 1,
+                                "sample": f"{Ms}x{N} (first {Ms} bases of seq1), 1 warm-up run, median of 3: "
+                                          f"{secs1:.2f} s"}}
     Ms = 16
     log(f"cpu baseline: oracle port on a {Ms}x{N} sample")
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -82,7 +114,7 @@ def cpu_baseline(M, N, s1, s2, threads):
     secs = time.time() - t0
     return {"value": round(B.cell_updates(Ms, N) / secs / 1e9, 3), "unit": "GCUPS", "cores": threads,
             "kind": "port", "sample": f"oracle restatement on the first {Ms} bases of seq1 x all {N} of seq2; "
-                                      f"{secs:.2f} s"}
+                                      f"{secs:.2f} s", "cpu": info}
 
 
 def parse_schedule(text):
@@ -261,13 +293,12 @@ def _pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else None
 
 
-def serving_leg(B, n_plain=160, n_sat=160, n_rate=160, fractions=(0.25, 0.5)):
-    """Continuous-batching server: saturated throughput and p50/p99 latency at Poisson
-    arrival rates below saturation, against the plain path (one solve call per pair) on the
-    same requests."""
+def serving_leg(B, eng, n_plain=160, n_sat=160, n_rate=160, fractions=(0.25, 0.5)):
+    """Continuous-batching server on the context `eng`: saturated throughput and p50/p99
+    latency at Poisson arrival rates below saturation, against the plain path (one solve call
+    per pair on the same context) on the same requests."""
     reqs = serving_requests(max(n_plain, n_sat, n_rate))
     cells = [B.cell_updates(len(a), len(b)) for a, b in reqs]
-    eng = B.Engine(0)
     try:
         log(f"serving: plain per-pair path on {n_plain} requests")
         eng.solve(*reqs[0], layout=B.LAYOUT_NONE)
@@ -329,11 +360,15 @@ def serving_leg(B, n_plain=160, n_sat=160, n_rate=160, fractions=(0.25, 0.5)):
                               "p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
                               "speedup_vs_plain": round(sat_rate / plain_rate, 3)},
                 "poisson": rates, "scores_identical_to_plain": same,
-                "server": {"budget_gb": round(st0["budget"] / 1e9, 1), "batches": st["batches"],
-                           "max_batch_seen": st["max_batch_seen"], "max_inflight_pairs": st["max_inflight_pairs"],
+                "server": {"budget_gb": round(st0["budget"] / 1e9, 1), "ticks": st["batches"],
+                           "max_pairs_per_tick": st["max_batch_seen"], "max_resident_pairs": st["max_inflight_pairs"],
                            "peak_gb_in_use": round(st["peak_bytes_in_use"] / 1e9, 2)}}
     finally:
-        eng.close()
+        if getattr(eng, "_server_tables", None) is not None:
+            try:
+                eng.server_stop()
+            except B.BPMaxError:
+                pass
 
 
 def main():
@@ -394,7 +429,7 @@ def main():
     if not args.no_extra:
         batched = batched_leg(B, eng)
         r0 = r0_leg(B, eng)
-        serving = serving_leg(B)
+        serving = serving_leg(B, eng)
 
     cpu = None
     if not args.no_cpu:

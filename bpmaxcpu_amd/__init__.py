@@ -103,6 +103,8 @@ def load_library(path: str = LIB_PATH):
         "bpmax_server_stop": (c_int, [c_void_p]),
         "bpmax_schedule_check": (c_int, [c_int, ip, ip, ctypes.POINTER(Tuning), c_int,
                                          ctypes.POINTER(ctypes.c_longlong), ctypes.c_char_p, c_int]),
+        "bpmax_server_check": (c_int, [c_int, ip, ip, ip, c_size_t, c_int, c_int, ctypes.POINTER(Tuning), c_int,
+                                       ctypes.POINTER(ctypes.c_longlong), ctypes.c_char_p, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -142,6 +144,24 @@ def schedule_check(shapes, tuning: dict, skip: int = -1):
     P = ctypes.POINTER(ctypes.c_int)
     rc = L.bpmax_schedule_check(len(shapes), Ms.ctypes.data_as(P), Ns.ctypes.data_as(P), ctypes.byref(t), skip,
                                 ctypes.byref(nchk), msg, len(msg))
+    return rc, nchk.value, msg.value.decode()
+
+
+def server_check(shapes, arrive, tuning: dict, memory_budget: int, max_active: int = 256, depth: int = 3,
+                 skip: int = -1):
+    """Host-only check of the serving loop's plan for requests of these (M, N) shapes arriving
+    at these tick counts (bpmax_server_check): returns (rc, dependencies checked, message)."""
+    L = load_library()
+    Ms = np.array([m for m, _ in shapes], np.int32)
+    Ns = np.array([n for _, n in shapes], np.int32)
+    At = np.array(arrive, np.int32)
+    t = Tuning.from_dict(tuning)
+    nchk = ctypes.c_longlong(0)
+    msg = ctypes.create_string_buffer(256)
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.bpmax_server_check(len(shapes), Ms.ctypes.data_as(P), Ns.ctypes.data_as(P), At.ctypes.data_as(P),
+                              memory_budget, max_active, depth, ctypes.byref(t), skip, ctypes.byref(nchk), msg,
+                              len(msg))
     return rc, nchk.value, msg.value.decode()
 
 

@@ -586,8 +586,7 @@ static int pick_chunk(long long total_steps, long long tiles)
 // Every range / DL position gets a Launch entry, possibly empty, so that a runner can
 // attach its events to each.  An R0 launch's `diag` is the last storage diagonal of its
 // range; launches carry the tick index in `d1`.
-static void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g, int T, int nblmax,
-                    Schedule &S)
+void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g, int T, int nblmax, Schedule &S)
 {
     auto emit = [&](int kind, size_t first, int diag) {
         S.launches.push_back(Launch{kind, T, true, (int)first, (int)(S.items.size() - first), 0, g, -1, diag});
@@ -926,6 +925,12 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
                                       L.count <= ctx->eff_intra_wide, ctx->intra_blocked != 0);
         });
     }
+}
+
+int bpx_run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDesc *dP, const WorkItem *dI,
+                   float *arena, const int *dcodes, const float *dw)
+{
+    return run_launch(ctx, st, L, dP, dI, arena, dcodes, dw);
 }
 
 // Dependency events of the pipelined schedule: fill, S2 done, end, then per T group one

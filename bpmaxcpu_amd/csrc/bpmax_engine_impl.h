@@ -65,6 +65,8 @@ size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
                     bool closure_ahead);
+// S2 of the pairs with sel[p] set, group g (tile size T), logical diagonals < nblmax.
+void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g, int T, int nblmax, Schedule &S);
 // One step of the outer wavefront for group g: pair p on outer diagonal d1s[p] (< 0: not in
 // the tick); out_end receives the last storage diagonal of the R0 range holding each Ds.
 void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nbmax, int nblmax,
@@ -134,6 +136,11 @@ struct bpmax_ctx {
     } while (0)
 
 int bpx_grow(bpmax_ctx *ctx, bpx::DevBuf &b, size_t bytes);
+// One launch of a schedule on stream st (nothing for an empty position); dP / dI / dcodes /
+// dw: device descriptors, work items, codes base (PairDesc.a/.b are int offsets from it)
+// and the 50 weights.
+int bpx_run_launch(bpmax_ctx *ctx, hipStream_t st, const bpx::Launch &L, const bpx::PairDesc *dP,
+                   const bpx::WorkItem *dI, float *arena, const int *dcodes, const float *dw);
 // Lays out and solves the pairs; with sync = false only enqueues (the context's streams
 // hold the work; the host-side copies live in the context until its next run).
 int bpx_run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms, const int *const *seq2s,

@@ -107,18 +107,21 @@ int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s
 int bpmax_engine_fetch(bpmax_ctx *ctx, int pair, bpmax_layout layout, float *F_out);
 
 /* ---- serving loop ---------------------------------------------------------------------- */
-/* Continuous batching over one device arena of `memory_budget` bytes.  Submitted pairs
- * queue FIFO; a server thread admits the queue head into a batch whenever a batch slot
- * is free and the head's F footprint (the tiled layout of bpmax_engine_pair_device_bytes,
- * ~M(M+1)N(N+1)/4 x 4 B) fits a free extent of the arena, launches the batch
- * asynchronously, and keeps up to `slots` batches in flight at once, so requests that
- * arrive while others run are admitted as slots and memory free up and their work
- * overlaps on the device.  Results are bit-identical to bpmax_engine_solve of the same
- * pair.  No counterpart in the reference (its bpmax() solves one pair per call). */
+/* Continuous batching over one device arena of `memory_budget` bytes.  The server thread
+ * advances every resident pair by one outer diagonal per "tick" -- one step of the outer
+ * wavefront for all resident pairs at once, each on its own diagonal -- and enqueues the
+ * ticks on two streams of its own, `slots` ticks ahead of the device.  Submitted pairs queue
+ * FIFO and join at the next tick that has room: room in the arena for the pair's extent (its
+ * tiled F layout, S1, S2 and codes, ~M(M+1)N(N+1)/4 x 4 B, bpmax_engine_pair_device_bytes),
+ * fewer than `max_batch` resident pairs, and room in the tick's work-item staging.  Requests
+ * that arrive while others run therefore start while those run, and pairs leave after their
+ * last diagonal.  Every device buffer is allocated by bpmax_server_start; the serving loop
+ * never allocates or frees device memory.  Results are bit-identical to bpmax_engine_solve
+ * of the same pair.  No counterpart in the reference (its bpmax() solves one pair per call). */
 typedef struct {
     size_t memory_budget; /* arena bytes; 0 = 80% of the free device memory at start */
-    int max_batch;        /* pairs per batch (0 = 64)                                  */
-    int slots;            /* batches in flight at once (0 = 3)                         */
+    int max_batch;        /* most pairs resident at once (0 = 256)                    */
+    int slots;            /* ticks enqueued ahead of the device, 1..8 (0 = 3)         */
     int queue_capacity;   /* queued requests before submit returns BPMAX_EBUSY (0 = 4096) */
     int keep_tables;      /* 1: F tables stay resident until bpmax_server_release and can be
                              read with bpmax_server_fetch; 0: freed when the score is ready */
@@ -127,9 +130,10 @@ typedef struct {
 } bpmax_server_config;
 
 typedef struct {
-    long long submitted, completed, failed, batches;
-    long long max_batch_seen;     /* largest batch launched                         */
-    long long max_inflight_pairs; /* most pairs on the device at one time           */
+    long long submitted, completed, failed;
+    long long batches;            /* ticks enqueued                                 */
+    long long max_batch_seen;     /* most pairs in one tick                         */
+    long long max_inflight_pairs; /* most pairs resident at one time                */
     size_t budget, bytes_in_use, peak_bytes_in_use;
 } bpmax_server_stats;
 
@@ -147,9 +151,10 @@ int bpmax_server_fetch(bpmax_ctx *ctx, long long id, bpmax_layout layout, float 
 /* Forget a finished request (and free its table when it was the last of its batch). */
 int bpmax_server_release(bpmax_ctx *ctx, long long id);
 int bpmax_server_get_stats(bpmax_ctx *ctx, bpmax_server_stats *stats);
-/* Finish every queued request, then stop the server thread and free the arena. */
+/* Finish every queued request, then stop the server thread and free the arena.  Queued
+ * requests that can never be admitted because unreleased kept tables hold the arena fail
+ * with BPMAX_ENOMEM (reported by poll/wait) instead of blocking the stop. */
 int bpmax_server_stop(bpmax_ctx *ctx);
-
 /* ---- drop-in compatibility symbol ----------------------------------------------- */
 /* Exact signature of the reference entry point (src/bpmax.c:129).  Computes on the GPU
  * and writes the caller's 6-level F pointer tree
@@ -292,6 +297,19 @@ int bpmax_ctx_synchronize(bpmax_ctx *ctx);
  * No counterpart in the reference (its OpenMP schedule is implicit in its loops). */
 int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, const bpmax_tuning *t, int skip,
                          long long *checked, char *msg, int msglen);
+/* Host-only check of the serving loop's plan (no GPU needed): replays `nreq` requests of
+ * shapes Ms[k] x Ns[k], request k arriving once the server has planned arrive[k] ticks,
+ * through the server's own admission and tick planner with the given budget, resident-pair
+ * limit, ticks in flight and launch settings t, then runs the resulting stream operations
+ * through a model of two FIFO streams with events and verifies that every work item starts
+ * after all the work it reads, every event wait names an event recorded before it (and not
+ * re-recorded since), resident extents never overlap and every operation on a pair finishes
+ * before its extent is released.  skip >= 0 drops that event wait first (to test the check).
+ * *checked: dependencies verified.  BPMAX_EINVAL with a message on a violation.  No
+ * counterpart in the reference. */
+int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const int *arrive, size_t memory_budget,
+                       int max_active, int depth, const bpmax_tuning *t, int skip, long long *checked,
+                       char *msg, int msglen);
 /* Cell updates of the full recurrence for an MxN pair:
  *   C(M+1,3)C(N+1,3) + 2C(M+1,2)C(N+1,3) + 2C(M+1,3)C(N+1,2)  (R0 + R1/R2 + R3/R4). */
 double bpmax_cell_updates(int M, int N);

@@ -165,8 +165,9 @@ def write_input(path, seq1, seq2, intra, inter):
 
 
 def reference_run(seq1, seq2, intra=None, inter=None, table=True, repeat=1, threads=None,
-                  workdir=None):
-    """Run the reference's bpmax() through the harness.
+                  workdir=None, warmup=0, env_extra=None):
+    """Run the reference's bpmax() through the harness: `warmup` untimed runs, then the
+    median of `repeat` timed ones.  env_extra adds environment variables (OMP_PROC_BIND ...).
 
     Returns (F_packed or None, score, median_seconds, n_tile)."""
     if not have_reference():
@@ -176,10 +177,11 @@ def reference_run(seq1, seq2, intra=None, inter=None, table=True, repeat=1, thre
     env = dict(os.environ)
     if threads is not None:
         env["OMP_NUM_THREADS"] = str(threads)
+    env.update(env_extra or {})
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         write_input(fin, seq1, seq2, intra, inter)
-        args = [REF_RUN, fin, fout, "--repeat", str(repeat)]
+        args = [REF_RUN, fin, fout, "--repeat", str(repeat), "--warmup", str(warmup)]
         if not table:
             args.append("--no-table")
         subprocess.run(args, check=True, env=env, stdout=subprocess.DEVNULL)

@@ -1,7 +1,10 @@
 """GPU tests of the continuous-batching server: every result is bit-identical to the same
-pair solved alone, batches overlap, admission respects the memory budget, and the error
-paths (too big, queue full, unknown id) report the documented codes."""
+pair solved alone and to the reference's tables for the serving workload's fixture set,
+pairs run side by side, admission respects the memory budget, stop() never blocks on
+kept tables, and the error paths (too big, queue full, unknown id) report the documented
+codes."""
 import sys
+import threading
 import time
 
 import numpy as np
@@ -9,6 +12,7 @@ import pytest
 
 import bpmaxcpu_amd as B
 from oracle import oracle as O
+from tests import golden_io as G
 
 pytestmark = pytest.mark.gpu
 
@@ -75,10 +79,6 @@ def test_server_results_match_solo_solves():
         solo.close()
 
 
-@pytest.mark.skip(reason="stopped making progress on the GPU in two runs (with a third inner stream, then with a "
-                         "one-wait-per-launch enqueue, both since removed); with the current enqueue a throughput "
-                         "probe with 3 slots and batches of <= 8 pairs stalled the same way; no cause found in the "
-                         "server or the pipelined enqueue, so the case is not run until one is")
 def test_server_overlaps_batches_and_respects_budget():
     """A budget of about three mid-size pairs: batches must run side by side, the arena
     in use never exceeds the budget, and every score still matches a solo solve."""
@@ -88,7 +88,7 @@ def test_server_overlaps_batches_and_respects_budget():
     srv = B.Engine(0)
     solo = B.Engine(0)
     try:
-        _say(f"server: 16 requests, budget {budget >> 20} MiB")
+        _say(f"server: 16 requests, budget {budget >> 20} MiB, at most 2 resident, 3 ticks ahead")
         srv.server_start(memory_budget=budget, max_batch=2, slots=3)
         ids = [srv.server_submit(a, b) for a, b in reqs]
         res = [srv.server_wait(i, 60000.0) for i in ids]
@@ -132,6 +132,69 @@ def test_server_error_codes():
         with pytest.raises(B.BPMaxError) as e:
             srv.server_poll(123456789)
         assert e.value.code == B.ENOTFOUND
+    finally:
+        srv.server_stop()
+        srv.close()
+
+
+def test_server_stop_does_not_block_on_kept_tables():
+    """With keep_tables, an unreleased table can hold the arena so that a queued pair never
+    fits: stop() must still return (the queued request fails with BPMAX_ENOMEM instead of
+    waiting for a release that cannot come while stop blocks the caller)."""
+    s1, s2 = np.zeros(12, np.int32), np.zeros(200, np.int32)
+    one = B.load_library().bpmax_engine_pair_device_bytes(12, 200)
+    srv = B.Engine(0)
+    try:
+        _say("server: a kept table blocks a second request; stop")
+        srv.server_start(memory_budget=one + (one >> 1), max_batch=4, slots=2, keep_tables=True)
+        a = srv.server_submit(s1, s2)
+        assert srv.server_wait(a, 60000.0) is not None
+        b = srv.server_submit(s1, s2)  # fits only once `a` is released
+        time.sleep(0.05)
+        assert srv.server_poll(b) is None
+        st = srv.server_stats()
+        assert st["completed"] == 1 and st["submitted"] == 2
+        stopper = threading.Thread(target=srv.server_stop)
+        stopper.start()
+        stopper.join(30.0)
+        assert not stopper.is_alive(), "server_stop blocked on a request that can never be admitted"
+    finally:
+        srv.close()
+
+
+@pytest.mark.parametrize("name", G.set_fixtures("serve5"))
+def test_server_serving_fixture_set_poisson_arrivals(name):
+    """The serving workload's fixture set (M log-uniform [8,64], N log-uniform [64,1024],
+    including 64x1024) submitted with Poisson-timed arrivals, tables kept: every score, the
+    strided cells and the SHA-256 of every fetched F table equal the reference's."""
+    fx = G.load_set(name)
+    pairs = fx["pairs"]
+    rng = np.random.default_rng(99)
+    gaps = rng.exponential(0.02, len(pairs))
+    srv = B.Engine(0)
+    try:
+        _say(f"server: {name}, {len(pairs)} requests, Poisson arrivals (mean gap 20 ms), tables kept")
+        srv.server_start(keep_tables=True, intra=fx["intra"], inter=fx["inter"])
+        ids = []
+        for p, g in zip(pairs, gaps):
+            time.sleep(float(g))
+            ids.append(srv.server_submit(p["seq1"], p["seq2"]))
+        for k, (p, i) in enumerate(zip(pairs, ids)):
+            r = srv.server_wait(i, 120000.0)
+            assert r is not None, k
+            assert r[0] == p["score"], (k, p["M"], p["N"])
+        st = srv.server_stats()
+        assert st["completed"] == len(pairs) and st["failed"] == 0
+        assert st["max_inflight_pairs"] >= 2
+        for k, (p, i) in enumerate(zip(pairs, ids)):
+            _say(f"server: fetch {k} {p['M']}x{p['N']}")
+            F = srv.server_fetch(i, p["M"], p["N"])
+            for i1, j1, i2, j2, v in p["samples"]:
+                assert F[O.packed_index(i1, j1, i2, j2, p["M"], p["N"])] == v
+            assert G.sha256_f32(F) == p["sha256"], (k, p["M"], p["N"])
+            del F
+            srv.server_release(i)
+        assert srv.server_stats()["bytes_in_use"] == 0
     finally:
         srv.server_stop()
         srv.close()
