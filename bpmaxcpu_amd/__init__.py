@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH):
         "bpmax_engine_solve": (c_int, [c_void_p, ip, c_int, ip, c_int, fp, fp, fp, c_int, fp]),
         "bpmax_engine_solve_batch": (c_int, [c_void_p, c_int, ipp, ip, ipp, ip, fp, fp, fp]),
         "bpmax_engine_fetch": (c_int, [c_void_p, c_int, c_int, fp]),
+        "bpmax_engine_solve_noalloc": (c_int, [c_void_p, ip, c_int, ip, c_int, fp, fp, fp, c_int, fp]),
         "bpmax_layout_bytes": (c_size_t, [c_int, c_int, c_int]),
         "bpmax_engine_pair_device_bytes": (c_size_t, [c_int, c_int]),
         "bpmax_tri_tiled_floats": (c_size_t, [c_int]),
@@ -264,6 +265,22 @@ class Engine:
         rc = self.L.bpmax_engine_solve(self.h, _ip(s1), M, _ip(s2), N,
                                        None if ti is None else _fp(ti), None if te is None else _fp(te),
                                        None if out is None else _fp(out), layout, ctypes.byref(score))
+        self._check(rc)
+        return out, float(score.value)
+
+    def solve_noalloc(self, seq1, seq2, intra=None, inter=None, layout: int = LAYOUT_PACKED):
+        """As solve() (LAYOUT_PACKED or LAYOUT_NONE) without any device allocation, in the
+        library's fixed per-GPU scratch (bpmax_engine_solve_noalloc)."""
+        s1 = np.ascontiguousarray(seq1, np.int32)
+        s2 = np.ascontiguousarray(seq2, np.int32)
+        M, N = len(s1), len(s2)
+        ti = _table(intra, None)
+        te = _table(inter, ti)
+        out = np.empty((M * (M + 1) // 2) * (N * (N + 1) // 2), np.float32) if layout == LAYOUT_PACKED else None
+        score = ctypes.c_float()
+        rc = self.L.bpmax_engine_solve_noalloc(self.h, _ip(s1), M, _ip(s2), N,
+                                               None if ti is None else _fp(ti), None if te is None else _fp(te),
+                                               None if out is None else _fp(out), layout, ctypes.byref(score))
         self._check(rc)
         return out, float(score.value)
 

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1374,16 +1375,34 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         P[p] = d;
     }
     int rc;
-    if (ctx->arena_ext) {
-        if ((size_t)fl * sizeof(float) > ctx->arena_ext_bytes) {
-            ctx->err = "pairs need " + std::to_string((size_t)fl * sizeof(float)) + " arena bytes, the slice holds " +
-                       std::to_string(ctx->arena_ext_bytes);
+    char *codes_dev = nullptr, *meta_dev = nullptr;
+    size_t meta_room = 0;
+    auto al256 = [](size_t x) { return (x + 255) / 256 * 256; };
+    if (ctx->carve) { // no-allocation mode: arena | codes | meta carved from the caller's region
+        const size_t ab = al256((size_t)fl * sizeof(float)), cb = al256((size_t)il * sizeof(int));
+        if (ab + cb + 256 > ctx->carve_bytes) {
+            ctx->err = "the pair needs " + std::to_string(ab + cb) + " bytes of device scratch for its tables, more "
+                       "than the " + std::to_string(ctx->carve_bytes) + " available";
             return BPMAX_ENOMEM;
         }
-    } else if ((rc = bpx_grow(ctx, ctx->arena, (size_t)fl * sizeof(float))) != BPMAX_OK) {
-        return rc;
+        ctx->arena_ext = ctx->carve;
+        ctx->arena_ext_bytes = ab;
+        codes_dev = ctx->carve + ab;
+        meta_dev = codes_dev + cb;
+        meta_room = ctx->carve_bytes - ab - cb;
+    } else {
+        if (ctx->arena_ext) {
+            if ((size_t)fl * sizeof(float) > ctx->arena_ext_bytes) {
+                ctx->err = "pairs need " + std::to_string((size_t)fl * sizeof(float)) +
+                           " arena bytes, the slice holds " + std::to_string(ctx->arena_ext_bytes);
+                return BPMAX_ENOMEM;
+            }
+        } else if ((rc = bpx_grow(ctx, ctx->arena, (size_t)fl * sizeof(float))) != BPMAX_OK) {
+            return rc;
+        }
+        if ((rc = bpx_grow(ctx, ctx->codes, (size_t)il * sizeof(int))) != BPMAX_OK) return rc;
+        codes_dev = (char *)ctx->codes.p;
     }
-    if ((rc = bpx_grow(ctx, ctx->codes, (size_t)il * sizeof(int))) != BPMAX_OK) return rc;
 
     std::vector<int> &codes = ctx->codes_host;
     codes.assign((size_t)il, 4);
@@ -1413,6 +1432,11 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         }
         auto tu = ctx->tuning.find(tuning_class(mM, mN, npairs));
         if (tu != ctx->tuning.end()) eff = tu->second;
+    }
+    // no-allocation mode: no graph capture, no task-scheduler counters
+    if (ctx->carve) {
+        if (eff.schedule == BPMAX_SCHED_TASKS) eff.schedule = BPMAX_SCHED_PIPE;
+        eff.graphs = 0;
     }
     const int sched = eff.schedule;
     ctx->eff_intra_wide = eff.intra_wide;
@@ -1453,26 +1477,38 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
     const size_t wbytes = 256;
     const size_t pbytes = (P.size() * sizeof(PairDesc) + 255) / 256 * 256;
     const size_t ibytes = S.items.size() * sizeof(WorkItem);
-    const size_t meta_cap = ctx->meta.cap;
-    if ((rc = bpx_grow(ctx, ctx->meta, wbytes + pbytes + ibytes)) != BPMAX_OK) return rc;
-    if (ctx->meta.cap != meta_cap) ctx->items_on_device = nullptr;
-    const bool send_items = ctx->items_on_device != ctx->meta.p;
+    bool send_items = true;
+    if (ctx->carve) {
+        if (wbytes + pbytes + ibytes > meta_room) {
+            ctx->err = "the schedule needs " + std::to_string(wbytes + pbytes + ibytes) + " bytes of device scratch "
+                       "beyond the tables, " + std::to_string(meta_room) + " are left";
+            return BPMAX_ENOMEM;
+        }
+        ctx->items_on_device = nullptr; // the shared scratch keeps nothing between calls
+    } else {
+        const size_t meta_cap = ctx->meta.cap;
+        if ((rc = bpx_grow(ctx, ctx->meta, wbytes + pbytes + ibytes)) != BPMAX_OK) return rc;
+        if (ctx->meta.cap != meta_cap) ctx->items_on_device = nullptr;
+        meta_dev = (char *)ctx->meta.p;
+        send_items = ctx->items_on_device != ctx->meta.p;
+    }
+    ctx->meta_dev = meta_dev;
     ctx->staging.resize(wbytes + pbytes + (send_items ? ibytes : 0));
     memcpy(ctx->staging.data(), wt, sizeof wt);
     memcpy(ctx->staging.data() + wbytes, P.data(), P.size() * sizeof(PairDesc));
     if (send_items) memcpy(ctx->staging.data() + wbytes + pbytes, S.items.data(), ibytes);
-    char *meta = (char *)ctx->meta.p;
+    char *meta = meta_dev;
     const float *dw = (const float *)meta;
     const PairDesc *dP = (const PairDesc *)(meta + wbytes);
     const WorkItem *dI = (const WorkItem *)(meta + wbytes + pbytes);
     float *arena = bpx_arena(ctx);
-    const int *dcodes = (const int *)ctx->codes.p;
+    const int *dcodes = (const int *)codes_dev;
 
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->meta.p, ctx->staging.data(), ctx->staging.size(), hipMemcpyHostToDevice,
+    HIPCHK(ctx, hipMemcpyAsync(meta_dev, ctx->staging.data(), ctx->staging.size(), hipMemcpyHostToDevice,
                                ctx->stream));
-    ctx->items_on_device = ctx->meta.p;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->codes.p, codes.data(), codes.size() * sizeof(int), hipMemcpyHostToDevice,
+    if (!ctx->carve) ctx->items_on_device = ctx->meta.p;
+    HIPCHK(ctx, hipMemcpyAsync(codes_dev, codes.data(), codes.size() * sizeof(int), hipMemcpyHostToDevice,
                                ctx->stream));
     hipStream_t st = ctx->stream;
 
@@ -1509,7 +1545,7 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
     }
 
     if (sched == BPMAX_SCHED_PIPE) {
-        if (eff.graphs && !ctx->timing)
+        if (eff.graphs && !ctx->timing && !ctx->carve)
             rc = launch_pipe_graph(ctx, key, npairs, S, dP, dI, arena, fl, dcodes, dw);
         else
             rc = enqueue_pipe(ctx, npairs, S, dP, dI, arena, fl, dcodes, dw);
@@ -1534,11 +1570,15 @@ int bpx_gather_scores(bpmax_ctx *ctx, float *scores, bool sync)
 {
     const int n = (int)ctx->pairs_host.size();
     int rc;
-    if ((rc = bpx_grow(ctx, ctx->scores, n * sizeof(float))) != BPMAX_OK) return rc;
+    float *dscores = ctx->carve_scores;
+    if (!dscores) {
+        if ((rc = bpx_grow(ctx, ctx->scores, n * sizeof(float))) != BPMAX_OK) return rc;
+        dscores = (float *)ctx->scores.p;
+    }
     const size_t wbytes = 256;
-    const PairDesc *dP = (const PairDesc *)((char *)ctx->meta.p + wbytes);
-    HIPCHK(ctx, launch_scores(ctx->stream, dP, n, bpx_arena(ctx), (float *)ctx->scores.p));
-    HIPCHK(ctx, hipMemcpyAsync(scores, ctx->scores.p, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    const PairDesc *dP = (const PairDesc *)(ctx->meta_dev + wbytes);
+    HIPCHK(ctx, launch_scores(ctx->stream, dP, n, bpx_arena(ctx), dscores));
+    HIPCHK(ctx, hipMemcpyAsync(scores, dscores, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     if (sync) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return BPMAX_OK;
 }
@@ -1719,6 +1759,93 @@ extern "C" int bpmax_tri_download(bpmax_ctx *ctx, int n, int batch, const float 
     if ((rc = bpx_grow(ctx, ctx->out, bytes)) != BPMAX_OK) return rc;
     HIPCHK(ctx, launch_tiled_to_dense(ctx->stream, n, nb, T, batch, dev_tiled, (float *)ctx->out.p));
     HIPCHK(ctx, hipMemcpyAsync(host_dense, ctx->out.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BPMAX_OK;
+}
+
+// ------------------------------------------------------------ no-allocation entry point
+// Device scratch of bpmax_engine_solve_noalloc: one array per GPU, loaded with the library's
+// code object and held until the process ends.  In host code `scratch` is only a stand-in
+// for the symbol: its device address on each GPU comes from hipGetSymbolAddress (once per
+// device, with that device current) and only that address is passed on.
+static __device__ __align__(256) unsigned char scratch[BPMAX_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, char *> g_scratch_ptr; // device -> address of `scratch` there
+
+static int scratch_on_device(bpmax_ctx *ctx, char **out)
+{
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(ctx->device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return BPMAX_OK;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return BPMAX_EDEVICE;
+    }
+    g_scratch_ptr[ctx->device] = (char *)p;
+    *out = (char *)p;
+    return BPMAX_OK;
+}
+
+// Layout of the scratch for one call: the run's arena | codes | meta from the front (carved
+// in bpx_run_pairs), then the scores (256 B) and the packed F (when asked for) at the end.
+extern "C" int bpmax_engine_solve_noalloc(bpmax_ctx *ctx, const int *seq1, int M, const int *seq2, int N,
+                                          const float *intra, const float *inter, float *F_out,
+                                          bpmax_layout layout, float *score_out)
+{
+    if (!ctx) return BPMAX_EINVAL;
+    if (layout != BPMAX_LAYOUT_NONE && layout != BPMAX_LAYOUT_PACKED_TRI_TRI) {
+        ctx->err = "the no-allocation solve writes BPMAX_LAYOUT_NONE or BPMAX_LAYOUT_PACKED_TRI_TRI";
+        return BPMAX_EINVAL;
+    }
+    if ((layout != BPMAX_LAYOUT_NONE && !F_out) || M < 1 || N < 1) {
+        ctx->err = "lengths must be >= 1 and F_out set for a layout that writes F";
+        return BPMAX_EINVAL;
+    }
+    const size_t kBytes = BPMAX_NOALLOC_SCRATCH_BYTES;
+    const size_t out_bytes =
+        layout == BPMAX_LAYOUT_NONE ? 0 : (bpmax_layout_bytes(M, N, layout) + 255) / 256 * 256;
+    if (out_bytes + 512 > kBytes) {
+        ctx->err = "the packed F table of " + std::to_string(M) + "x" + std::to_string(N) + " needs " +
+                   std::to_string(out_bytes) + " bytes, more than the no-allocation scratch holds";
+        return BPMAX_ENOMEM;
+    }
+    char *base = nullptr;
+    int rc = scratch_on_device(ctx, &base);
+    if (rc != BPMAX_OK) return rc;
+    char *tail = base + kBytes - out_bytes - 256;
+    ctx->carve = base;
+    ctx->carve_bytes = kBytes - out_bytes - 256;
+    ctx->carve_scores = (float *)tail;
+    struct Restore { // the context's own buffers again after the call, whatever its outcome
+        bpmax_ctx *c;
+        ~Restore()
+        {
+            c->carve = nullptr;
+            c->carve_bytes = 0;
+            c->carve_scores = nullptr;
+            c->arena_ext = nullptr;
+            c->arena_ext_bytes = 0;
+            c->result_valid = false; // the shared scratch is not a resident result
+            c->items_on_device = nullptr;
+            c->meta_dev = nullptr;
+        }
+    } restore{ctx};
+    rc = bpx_run_pairs(ctx, 1, &seq1, &M, &seq2, &N, intra, inter, true);
+    if (rc != BPMAX_OK) return rc;
+    float score = 0.f;
+    if ((rc = bpx_gather_scores(ctx, &score, true)) != BPMAX_OK) return rc;
+    if (score_out) *score_out = score;
+    if (layout == BPMAX_LAYOUT_NONE) return BPMAX_OK;
+    float *dout = (float *)(tail + 256);
+    HIPCHK(ctx, launch_to_packed(ctx->stream, bpx_arena(ctx), ctx->pairs_host[0], dout));
+    HIPCHK(ctx, hipMemcpyAsync(F_out, dout, bpmax_layout_bytes(M, N, layout), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return BPMAX_OK;
 }

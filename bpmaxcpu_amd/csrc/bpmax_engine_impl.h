@@ -94,8 +94,14 @@ struct bpmax_ctx {
     void *items_on_device = nullptr;    // meta buffer that already holds sched.items
     std::vector<char> staging;
     std::vector<int> codes_host;
-    void *arena_ext = nullptr;           // device arena slice owned by someone else (server)
+    void *arena_ext = nullptr;           // device arena slice owned by someone else
     size_t arena_ext_bytes = 0;
+    // no-allocation mode (bpmax_engine_solve_noalloc): a run's arena, codes and meta are
+    // carved from this device region, its scores land in carve_scores
+    char *carve = nullptr;
+    size_t carve_bytes = 0;
+    float *carve_scores = nullptr;
+    char *meta_dev = nullptr;            // meta block (weights | descs | items) of the last run
     // pipelined schedule: second stream, dependency events, cached graph
     hipStream_t stream2 = nullptr;       // inner closure
     int outer_split = 2;                 // R0 launches per outer diagonal, pipelined schedule

@@ -106,6 +106,25 @@ int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s
  * reference (its bpmax() writes F into the caller's table directly). */
 int bpmax_engine_fetch(bpmax_ctx *ctx, int pair, bpmax_layout layout, float *F_out);
 
+/* ---- solve without device allocation ----------------------------------------------- */
+/* Size of the per-GPU device scratch array that bpmax_engine_solve_noalloc works in.  The
+ * array is part of the library's code object: each GPU the library runs on holds it until
+ * the process ends. */
+#define BPMAX_NOALLOC_SCRATCH_BYTES (64u << 20)
+/* As bpmax_engine_solve with layout BPMAX_LAYOUT_NONE or BPMAX_LAYOUT_PACKED_TRI_TRI, but it
+ * allocates no device memory (no hipMalloc/hipFree, no allocating library routine): the F
+ * table, codes, schedule, score and the packed copy of F all live in the scratch array
+ * above.  A pair whose tables, schedule and packed output do not fit its
+ * BPMAX_NOALLOC_SCRATCH_BYTES returns BPMAX_ENOMEM with a message (64 MiB holds, with the
+ * packed output, e.g. 24 x 160, 32 x 128 and 16 x 256).  Every call on one GPU shares that
+ * GPU's array and keeps nothing between calls (F is re-initialised by each call), so calls
+ * that overlap on one GPU -- from several threads or contexts -- race and must not be issued.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int bpmax_engine_solve_noalloc(bpmax_ctx *ctx, const int *seq1, int M, const int *seq2, int N,
+                               const float *intra, const float *inter, float *F_out,
+                               bpmax_layout layout, float *score_out);
+
 /* ---- serving loop ---------------------------------------------------------------------- */
 /* Continuous batching over one device arena of `memory_budget` bytes.  The server thread
  * advances every resident pair by one outer diagonal per "tick" -- one step of the outer
