@@ -118,11 +118,12 @@ def cpu_baseline(M, N, s1, s2, threads, single_M=32):
 
 
 def parse_schedule(text):
-    """'pipe:2:4096:0:64:2[:1]' -> tuning dict (schedule, outer_split, outer_items, graphs,
-    intra_wide, outer_kgroup[, closure_ahead])."""
+    """'pipe:2:4096:0:64:2[:1[:32]]' -> tuning dict (schedule, outer_split, outer_items, graphs,
+    intra_wide, outer_kgroup[, closure_ahead[, tile]])."""
     f = text.split(":")
     return {"schedule": f[0], "outer_split": int(f[1]), "outer_items": int(f[2]), "graphs": bool(int(f[3])),
-            "intra_wide": int(f[4]), "outer_kgroup": int(f[5]), "closure_ahead": len(f) > 6 and f[6] == "1"}
+            "intra_wide": int(f[4]), "outer_kgroup": int(f[5]), "closure_ahead": len(f) > 6 and f[6] == "1",
+            "tile": int(f[7]) if len(f) > 7 else 0}
 
 
 def pair_leg(B, eng, M, N, steps, warmup, schedule=None):

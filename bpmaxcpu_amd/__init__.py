@@ -119,18 +119,18 @@ class Tuning(ctypes.Structure):
     """bpmax_tuning: launch-schedule settings of one problem class."""
     _fields_ = [("schedule", ctypes.c_int), ("outer_split", ctypes.c_int), ("outer_items", ctypes.c_int),
                 ("graphs", ctypes.c_int), ("intra_wide", ctypes.c_int), ("outer_kgroup", ctypes.c_int),
-                ("closure_ahead", ctypes.c_int)]
+                ("closure_ahead", ctypes.c_int), ("tile", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {"schedule": {v: k for k, v in SCHEDULES.items()}[self.schedule], "outer_split": self.outer_split,
                 "outer_items": self.outer_items, "graphs": bool(self.graphs), "intra_wide": self.intra_wide,
-                "outer_kgroup": self.outer_kgroup, "closure_ahead": bool(self.closure_ahead)}
+                "outer_kgroup": self.outer_kgroup, "closure_ahead": bool(self.closure_ahead), "tile": self.tile}
 
     @staticmethod
     def from_dict(tuning: dict) -> "Tuning":
         return Tuning(SCHEDULES[tuning["schedule"]], tuning["outer_split"], tuning["outer_items"],
                       1 if tuning.get("graphs") else 0, tuning.get("intra_wide", 256), tuning.get("outer_kgroup", 1),
-                      1 if tuning.get("closure_ahead", True) else 0)
+                      1 if tuning.get("closure_ahead", True) else 0, tuning.get("tile", 0))
 
 
 def schedule_check(shapes, tuning: dict, skip: int = -1):

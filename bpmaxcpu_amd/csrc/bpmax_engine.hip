@@ -35,6 +35,8 @@ namespace bpx {
 const char *kClassNames[BPMAX_K_NCLASSES] = {"fill", "s1", "outer", "inner_ext", "inner_intra", "convert", "r0tri", "tasks"};
 
 int choose_tile(int N) { return N <= 64 ? 32 : 64; }
+// storage tile of an N-long strand under a tuning's `tile` setting (0: choose_tile)
+int pick_tile(int N, int tile) { return tile == 32 ? 32 : choose_tile(N); }
 
 unsigned long long splitmix64_host(unsigned long long x)
 {
@@ -244,7 +246,7 @@ static bool tuning_valid(const bpmax_tuning *t)
 {
     return t && (t->schedule == BPMAX_SCHED_PIPE || t->schedule == BPMAX_SCHED_LAUNCH) && t->outer_split >= 0 &&
            t->outer_items >= 64 && t->intra_wide >= 0 && t->outer_kgroup >= 0 && t->outer_kgroup <= 255 &&
-           (t->closure_ahead == 0 || t->closure_ahead == 1);
+           (t->closure_ahead == 0 || t->closure_ahead == 1) && (t->tile == 0 || t->tile == 32);
 }
 } // namespace bpx
 
@@ -257,8 +259,8 @@ extern "C" int bpmax_ctx_set_tuning(bpmax_ctx *ctx, int M, int N, int npairs, co
         return BPMAX_OK;
     }
     if (!tuning_valid(t)) return BPMAX_EINVAL;
-    ctx->tuning[cls] = Tuning{t->schedule,   t->outer_split,  t->outer_items,  t->graphs ? 1 : 0,
-                              t->intra_wide, t->outer_kgroup, t->closure_ahead};
+    ctx->tuning[cls] = Tuning{t->schedule,   t->outer_split,  t->outer_items,   t->graphs ? 1 : 0,
+                              t->intra_wide, t->outer_kgroup, t->closure_ahead, t->tile};
     return BPMAX_OK;
 }
 
@@ -272,6 +274,7 @@ extern "C" int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *
     ctx->intra_wide = t->intra_wide;
     ctx->outer_kgroup = t->outer_kgroup;
     ctx->closure_ahead = t->closure_ahead;
+    ctx->tile = t->tile;
     return BPMAX_OK;
 }
 
@@ -281,7 +284,7 @@ extern "C" int bpmax_ctx_get_tuning(bpmax_ctx *ctx, int M, int N, int npairs, bp
     auto it = ctx->tuning.find(tuning_class(M, N, npairs));
     if (it == ctx->tuning.end()) return BPMAX_ENOTFOUND;
     *t = bpmax_tuning{it->second.schedule,   it->second.outer_split,  it->second.outer_items,  it->second.graphs,
-                      it->second.intra_wide, it->second.outer_kgroup, it->second.closure_ahead};
+                      it->second.intra_wide, it->second.outer_kgroup, it->second.closure_ahead, it->second.tile};
     return BPMAX_OK;
 }
 
@@ -367,6 +370,14 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
         c.closure_ahead = ahead;
         rc = trial(c);
     }
+    // storage tile 32 instead of 64 (N > 64): less padding of N to whole tiles, against the
+    // 64-wide product's lower LDS traffic per update
+    const Tuning ahead0 = best;
+    if (rc == BPMAX_OK && mN > 64) {
+        Tuning c = ahead0;
+        c.tile = c.tile == 32 ? 0 : 32;
+        rc = trial(c);
+    }
     // graph replay of the pipelined schedule: it saves most on small problems, whose solves
     // are chains of short launches (16x100: 1.79 -> 1.48 ms)
     const Tuning ahead_best = best;
@@ -383,7 +394,7 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
     ctx->tuning[cls] = best;
     if (best_out)
         *best_out = bpmax_tuning{best.schedule,   best.outer_split,  best.outer_items,  best.graphs,
-                                 best.intra_wide, best.outer_kgroup, best.closure_ahead};
+                                 best.intra_wide, best.outer_kgroup, best.closure_ahead, best.tile};
     if (best_ms) *best_ms = best_t;
     return BPMAX_OK;
 }
@@ -597,19 +608,19 @@ void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g
             size_t f = S.items.size();
             long long tot = 0;
             for (size_t p = 0; p < P.size(); ++p)
-                if (sel[p] && P[p].T == T && P[p].nbl > D) tot += (long long)(P[p].nbl - D) * D;
+                if (sel[p] && P[p].T == T && P[p].nblr > D) tot += (long long)(P[p].nblr - D) * D;
             const int ch = pick_chunk(tot, 0);
             for (size_t p = 0; p < P.size(); ++p) {
-                if (!sel[p] || P[p].T != T || P[p].nbl <= D) continue;
-                for (int I = 0; I + D < P[p].nbl; ++I)
+                if (!sel[p] || P[p].T != T || P[p].nblr <= D) continue;
+                for (int I = 0; I + D < P[p].nblr; ++I)
                     chunked(S.items, WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0}, D, ch, false);
             }
             emit(BPMAX_K_INNER_EXT, f, D);
         }
         size_t f = S.items.size();
         for (size_t p = 0; p < P.size(); ++p) {
-            if (!sel[p] || P[p].T != T || P[p].nbl <= D) continue;
-            for (int I = 0; I + D < P[p].nbl; ++I) S.items.push_back(WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0});
+            if (!sel[p] || P[p].T != T || P[p].nblr <= D) continue;
+            for (int I = 0; I + D < P[p].nblr; ++I) S.items.push_back(WorkItem{(int)p, 0, 0, I, I + D, 0, 0, 0});
         }
         emit(BPMAX_K_INNER_INTRA, f, D);
     }
@@ -686,13 +697,13 @@ void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int 
             size_t f = S.items.size();
             long long tot = 0;
             for (size_t p = 0; p < P.size(); ++p)
-                if (active(p) && P[p].nbl > D) tot += (long long)(P[p].M - d1s[p]) * (P[p].nbl - D) * 2 * D;
+                if (active(p) && P[p].nblr > D) tot += (long long)(P[p].M - d1s[p]) * (P[p].nblr - D) * 2 * D;
             const int ch = pick_chunk(tot, 0);
             for (size_t p = 0; p < P.size(); ++p) {
-                if (!active(p) || P[p].nbl <= D) continue;
+                if (!active(p) || P[p].nblr <= D) continue;
                 const int d1 = d1s[p];
                 for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
-                    for (int I = 0; I + D < P[p].nbl; ++I)
+                    for (int I = 0; I + D < P[p].nblr; ++I)
                         chunked(S.items, WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, 0}, 2 * D, ch, false);
             }
             emit(BPMAX_K_INNER_EXT, f, 0, D);
@@ -706,19 +717,19 @@ void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int 
         const int D1 = D + 1;
         long long tot = 0;
         for (size_t p = 0; p < P.size() && closure_ahead; ++p)
-            if (active(p) && P[p].nbl > D1) tot += (long long)(P[p].M - d1s[p]) * (P[p].nbl - D1) * 2 * D;
+            if (active(p) && P[p].nblr > D1) tot += (long long)(P[p].M - d1s[p]) * (P[p].nblr - D1) * 2 * D;
         const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
         const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
         // by triangle, each triangle's in-tile items then its cross-tile items, so that every
         // XCD's contiguous share of the launch mixes the two kinds
         for (size_t p = 0; p < P.size(); ++p) {
-            if (!active(p) || P[p].nbl <= D) continue;
+            if (!active(p) || P[p].nblr <= D) continue;
             const int d1 = d1s[p];
             for (int i1 = 0; i1 + d1 < P[p].M; ++i1) {
-                for (int I = 0; I + D < P[p].nbl; ++I)
+                for (int I = 0; I + D < P[p].nblr; ++I)
                     S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + D, 0, 0, tin});
                 if (!closure_ahead || D == 0) continue;
-                for (int I = 0; I + D1 < P[p].nbl; ++I) {
+                for (int I = 0; I + D1 < P[p].nblr; ++I) {
                     const WorkItem base{(int)p, i1, i1 + d1, I, I + D1, 0, 0, tex};
                     chunked_range(S.items, base, 0, D, ch);
                     chunked_range(S.items, base, D + 2, 2 * D1, ch);
@@ -750,7 +761,7 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
         GroupInfo &G = S.groups[g];
         G.maxM = std::max(G.maxM, p.M);
         G.maxnb = std::max(G.maxnb, p.nb);
-        G.maxnbl = std::max(G.maxnbl, p.nbl);
+        G.maxnbl = std::max(G.maxnbl, p.nblr);
     }
     const std::vector<char> all(P.size(), 1);
     std::vector<int> d1s(P.size());
@@ -1120,9 +1131,10 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
         PairDesc d{};
         d.M = Ms[p];
         d.N = Ns[p];
-        d.T = choose_tile(d.N);
+        d.T = pick_tile(d.N, t->tile);
         d.nb = (d.N + d.T - 1) / d.T;
         d.nbl = d.nb * (d.T / inner_tile(d.T));
+        d.nblr = (d.N + inner_tile(d.T) - 1) / inner_tile(d.T);
         P[p] = d;
     }
     const bool one = t->schedule == BPMAX_SCHED_LAUNCH;
@@ -1214,8 +1226,8 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
     for (int p = 0; p < npairs; ++p)
         for (int i1 = 0; i1 < P[p].M; ++i1)
             for (int j1 = i1; j1 < P[p].M; ++j1)
-                for (int X = 0; X < P[p].nbl; ++X)
-                    for (int Y = X; Y < P[p].nbl; ++Y)
+                for (int X = 0; X < P[p].nblr; ++X)
+                    for (int Y = X; Y < P[p].nblr; ++Y)
                         if (!closed.count(Key{p, i1, j1, X, Y})) return fail("a tile is never closed");
     // cross-tile step coverage of F tiles: each of the 2D steps exactly once
     std::map<Key, std::vector<char>> steps_run;
@@ -1244,8 +1256,8 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
     for (int p = 0; p < npairs; ++p)
         for (int i1 = 0; i1 < P[p].M; ++i1)
             for (int j1 = i1; j1 < P[p].M; ++j1)
-                for (int X = 0; X < P[p].nbl; ++X)
-                    for (int Y = X + 1; Y < P[p].nbl; ++Y) {
+                for (int X = 0; X < P[p].nblr; ++X)
+                    for (int Y = X + 1; Y < P[p].nblr; ++Y) {
                         const auto it = steps_run.find(Key{p, i1, j1, X, Y});
                         if (it == steps_run.end() || std::count(it->second.begin(), it->second.end(), 1) != 2 * (Y - X))
                             return fail("a cross-tile step is never run");
@@ -1267,7 +1279,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                 for (int k1 = w.i1; k1 < w.j1; ++k1)
                     for (int side = 0; side < 2; ++side) {
                         const int a = side ? k1 + 1 : w.i1, b = side ? w.j1 : k1;
-                        const Key src{w.pair, a, b, w.I * rs, std::min((w.J + 1) * rs, p.nbl) - 1};
+                        const Key src{w.pair, a, b, w.I * rs, std::min((w.J + 1) * rs, p.nblr) - 1};
                         if (!before(closed.at(src), i)) return fail(std::string("R0 before the closure it reads") + where);
                         ++nchk;
                     }
@@ -1330,6 +1342,19 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
     memcpy(wt, intra_in ? intra_in : dflt_intra, 25 * sizeof(float));
     memcpy(wt + 25, inter_in ? inter_in : (intra_in ? intra_in : dflt_inter), 25 * sizeof(float));
 
+    // Launch-schedule settings: the tuned ones of this problem class, else the context's.
+    Tuning eff{ctx->schedule,   ctx->outer_split,  ctx->outer_items,  ctx->graphs ? 1 : 0,
+               ctx->intra_wide, ctx->outer_kgroup, ctx->closure_ahead, ctx->tile};
+    if (ctx->schedule != BPMAX_SCHED_TASKS) {
+        int mM = 0, mN = 0;
+        for (int p = 0; p < npairs; ++p) {
+            mM = std::max(mM, Ms[p]);
+            mN = std::max(mN, Ns[p]);
+        }
+        auto tu = ctx->tuning.find(tuning_class(mM, mN, npairs));
+        if (tu != ctx->tuning.end()) eff = tu->second;
+    }
+
     std::vector<PairDesc> &P = ctx->pairs_host;
     P.assign(npairs, PairDesc{});
     long long fl = 0, il = 0;
@@ -1358,9 +1383,10 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         PairDesc d{};
         d.M = M;
         d.N = N;
-        d.T = choose_tile(N);
+        d.T = pick_tile(N, eff.tile);
         d.nb = (N + d.T - 1) / d.T;
         d.nbl = d.nb * (d.T / inner_tile(d.T));
+        d.nblr = (d.N + inner_tile(d.T) - 1) / inner_tile(d.T);
         d.tstride = tri_ll(d.nb) * d.T * d.T;
         d.F = fl;
         fl += al(tri_ll(M) * d.tstride);
@@ -1421,18 +1447,6 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         if (std::find(Ts.begin(), Ts.end(), d.T) == Ts.end()) Ts.push_back(d.T);
     std::sort(Ts.begin(), Ts.end());
 
-    // Launch-schedule settings: the tuned ones of this problem class, else the context's.
-    Tuning eff{ctx->schedule,   ctx->outer_split,  ctx->outer_items,  ctx->graphs ? 1 : 0,
-               ctx->intra_wide, ctx->outer_kgroup, ctx->closure_ahead};
-    if (ctx->schedule != BPMAX_SCHED_TASKS) {
-        int mM = 0, mN = 0;
-        for (auto &d : P) {
-            mM = std::max(mM, d.M);
-            mN = std::max(mN, d.N);
-        }
-        auto tu = ctx->tuning.find(tuning_class(mM, mN, npairs));
-        if (tu != ctx->tuning.end()) eff = tu->second;
-    }
     // no-allocation mode: no graph capture, no task-scheduler counters
     if (ctx->carve) {
         if (eff.schedule == BPMAX_SCHED_TASKS) eff.schedule = BPMAX_SCHED_PIPE;
@@ -1443,7 +1457,7 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
 
     // The schedule is rebuilt only when the shapes or the schedule settings change.
     std::vector<long long> key = {sched,           ctx->task_chunk,   eff.outer_split, eff.outer_items,
-                                  eff.outer_kgroup, eff.closure_ahead, npairs};
+                                  eff.outer_kgroup, eff.closure_ahead, eff.tile,        npairs};
     // (intra_wide changes kernel choices, not the schedule: the graph key carries it)
     for (auto &d : P) {
         key.push_back(d.M);

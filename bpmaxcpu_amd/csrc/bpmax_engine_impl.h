@@ -32,6 +32,7 @@ struct Launch {
 // launch-schedule settings of one problem class (bpmax_tuning)
 struct Tuning {
     int schedule, outer_split, outer_items, graphs, intra_wide, outer_kgroup, closure_ahead;
+    int tile = 0; // storage tile: 0 = choose_tile, 32 = 32 for every pair
 };
 
 struct GroupInfo {
@@ -61,6 +62,7 @@ struct PendingTiming {
 struct Server; // continuous-batching server (bpmax_server.cpp)
 
 int choose_tile(int N);
+int pick_tile(int N, int tile);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
@@ -113,6 +115,7 @@ struct bpmax_ctx {
                                          // (or, with intra_blocked = 0, the 512-lane) form
     int eff_intra_wide = 256;            // the current run's threshold
     int closure_ahead = 1;               // inner closure in one launch per diagonal (bpmax_tuning)
+    int tile = 0;                        // storage tile setting (bpmax_tuning)
     int intra_blocked = 1;               // F in-tile launches that are not wide use the blocked
                                          // 256-lane form (0: the 512-lane push form)
     std::map<long long, bpx::Tuning> tuning; // tuned settings per problem class

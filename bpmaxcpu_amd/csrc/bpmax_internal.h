@@ -22,7 +22,9 @@ struct PairDesc {
     int nb;            // storage tiles per inner side
     int T;             // storage tile edge (all pairs of one launch share it)
     int nbl;           // logical tiles per inner side for the inner phases (edge min(T, 32))
-    int pad_;
+    int nblr;          // logical tiles per side that hold real cells, ceil(N / edge): tiles
+                       // (X, Y) with Y >= nblr lie wholly in the padding, which no real cell
+                       // reads, and are never closed
     long long F;       // float offset of triangle (0,0)
     long long tstride; // floats per outer triangle = tri(nb)*T*T
     long long S2;      // float offset of the tiled S2 triangle

@@ -126,6 +126,16 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
         for (int j = 0; j < R; ++j) acc2[i][j] = NEG_INF;
     int hr0 = 0, hc0 = 0;
     if (hw >= 0) lane_block<T>(hr0, hc0, hw * 64 + (int)(threadIdx.x % 64));
+    // Padding (N not a multiple of T): inner indices N.. of the last storage tile hold no real
+    // cell, and no real cell reads them -- a real cell (r, c) takes k2 in [r, c), so k2 <= N-2.
+    // On steps with K = nb-1 the product stops at the last real k (rounded up to H), and a wave
+    // whose quadrant lies wholly in padding rows (I = nb-1) or columns (J = nb-1) runs no
+    // product at all; the padding cells keep values no real cell uses.
+    const int kreal = (P.N - (nb - 1) * T + H - 1) / H * H; // <= T
+    auto quad_pad = [&](int w) {
+        return (I == nb - 1 && I * T + (w >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (w & 1) * QS >= P.N);
+    };
+    const bool own_pad = quad_pad(wv), help_pad = hw >= 0 && quad_pad(hw);
     for (int s = it.step_begin; s < it.step_end; ++s) {
         const int K = Kb + s % nK;
         __syncthreads();
@@ -149,8 +159,9 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
         }
         int kb = mb, ke = me;
         if (!balance) quad_k_range<T>(K == I, K == J, false, kb, ke);
-        product<T, H>(acc, As, Bs, r0, c0, kb, ke);
-        if (hw >= 0) product<T, H>(acc2, As, Bs, hr0, hc0, hb, he);
+        const int klim = K == nb - 1 ? kreal : T;
+        if (!own_pad) product<T, H>(acc, As, Bs, r0, c0, kb, ke < klim ? ke : klim);
+        if (hw >= 0 && !help_pad) product<T, H>(acc2, As, Bs, hr0, hc0, hb, he < klim ? he : klim);
     }
     if (balance) { // hand the slices over through the (now free) operand LDS
         constexpr int RG = R * R * 64;

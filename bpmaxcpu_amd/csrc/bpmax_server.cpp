@@ -124,6 +124,7 @@ static PairDesc layout_extent(int M, int N, long long e)
     d.T = choose_tile(N);
     d.nb = (N + d.T - 1) / d.T;
     d.nbl = d.nb * (d.T / inner_tile(d.T));
+    d.nblr = (N + inner_tile(d.T) - 1) / inner_tile(d.T);
     d.tstride = tri_ll(d.nb) * d.T * d.T;
     d.F = e;
     d.S2 = d.F + (long long)al64((size_t)(tri_ll(M) * d.tstride));
@@ -334,8 +335,8 @@ struct ServeCore {
             for (size_t k = 0; k < P.size(); ++k)
                 if (P[k].T == G.T) {
                     G.nbmax = std::max(G.nbmax, P[k].nb);
-                    G.nblmax = std::max(G.nblmax, P[k].nbl);
-                    if (sel[k]) s2nbl = std::max(s2nbl, P[k].nbl);
+                    G.nblmax = std::max(G.nblmax, P[k].nblr);
+                    if (sel[k]) s2nbl = std::max(s2nbl, P[k].nblr);
                 }
             if (s2nbl) emit_s2(P, sel, g, G.T, s2nbl, tp.S);
         }
@@ -1093,7 +1094,7 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
             if (o.kind == OP_SCORES) {
                 for (int f : tp.fin) {
                     const PairDesc &d = tp.P[f];
-                    const auto c = closed.find(Key{tp.ids[f], 0, d.M - 1, 0, d.nbl - 1});
+                    const auto c = closed.find(Key{tp.ids[f], 0, d.M - 1, 0, d.nblr - 1});
                     if (c == closed.end() || !before(c->second, i)) return fail("score read before its cell is final");
                     ++nchk;
                 }
@@ -1116,7 +1117,7 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
                     if (!before(s2_last.at(id), i) || !before(s1_op.at(id), i))
                         return fail(std::string("R0 before S1/S2") + where);
                     auto src_closed = [&](int a, int b) {
-                        const auto c = closed.find(Key{id, a, b, it.I * rs, std::min((it.J + 1) * rs, d.nbl) - 1});
+                        const auto c = closed.find(Key{id, a, b, it.I * rs, std::min((it.J + 1) * rs, d.nblr) - 1});
                         return c != closed.end() && before(c->second, i);
                     };
                     for (int k1 = it.i1; k1 < it.j1; ++k1) {
@@ -1160,7 +1161,7 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
     // every tile of every pair closed; every operation on a pair done before its release
     for (int q = 0; q < nreq; ++q) {
         const Request &r = C.reqs[rid[q]];
-        const int T = choose_tile(r.N), nbl = (r.N + T - 1) / T * (T / inner_tile(T));
+        const int nbl = r.desc.nblr;
         for (int i1 = 0; i1 < r.M; ++i1)
             for (int j1 = i1; j1 < r.M; ++j1)
                 for (int X = 0; X < nbl; ++X)

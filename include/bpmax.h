@@ -287,6 +287,8 @@ typedef struct {
                         the other cross-tile steps of DL+1 riding in the same launch (half
                         as many dependent launches); 0: a cross-tile and an in-tile launch
                         per diagonal                                                     */
+    int tile;        /* storage tile edge: 0 (default) = 32 for N <= 64, else 64; 32 = 32 for
+                        every pair (less padding of N to whole tiles, a 32 x 32 R0 product)   */
 } bpmax_tuning;
 /* Time every candidate setting on `npairs` synthetic M x N pairs (median of `reps` runs
  * after a warm-up each), keep the fastest for the class and return it. */

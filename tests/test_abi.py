@@ -137,6 +137,8 @@ def ctypes_ref(x):
     {"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "outer_kgroup": 1, "closure_ahead": True},
     {"schedule": "pipe", "outer_split": 0, "outer_items": 64, "outer_kgroup": 0, "closure_ahead": True},
     {"schedule": "launch", "outer_split": 1, "outer_items": 2048, "outer_kgroup": 2, "closure_ahead": True},
+    {"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "outer_kgroup": 1, "closure_ahead": True, "tile": 32},
+    {"schedule": "pipe", "outer_split": 0, "outer_items": 512, "outer_kgroup": 2, "closure_ahead": False, "tile": 32},
 ])
 def test_schedule_runs_every_item_after_what_it_reads(tuning):
     """The launch list of a 40x300 pair and of a mixed batch (two tile sizes, ragged M and N)

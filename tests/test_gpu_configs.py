@@ -28,10 +28,13 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module", params=["tuned-pipe", "launch"])
+@pytest.fixture(scope="module", params=["tuned-pipe", "launch", "tile32"])
 def eng(request):
     e = B.Engine(0)
-    if request.param == "launch":
+    if request.param == "tile32":
+        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": False,
+                              "intra_wide": 256, "outer_kgroup": 1, "closure_ahead": True, "tile": 32})
+    elif request.param == "launch":
         e.set_default_tuning({"schedule": "launch", "outer_split": 1, "outer_items": 2048, "graphs": False,
                               "intra_wide": 256, "outer_kgroup": 1, "closure_ahead": True})
         e.set_schedule("launch")

@@ -31,8 +31,8 @@ def eng():
 
 
 @pytest.fixture(scope="module",
-                params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "launch-narrow", "launch-wide", "launch-ahead",
-                        "tasks"])
+                params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "pipe-t32", "launch-narrow", "launch-wide",
+                        "launch-ahead", "tasks"])
 def eng2(request):
     """An engine under each schedule:
     * pipe: the two-stream pipeline with the default settings (inner closure in one launch
@@ -44,6 +44,8 @@ def eng2(request):
     * launch-*: per-phase launches on one stream with the in-tile closure always blocked
       (narrow; R0 items over every K of a tile), always 1024-lane (wide; R0 items of two K
       each, the last one ragged) or in one launch per diagonal (ahead);
+    * pipe-t32: the default pipeline with 32 x 32 storage tiles for every N (the tuner's
+      tile candidate);
     * tasks: the dependency-tile scheduler."""
     push = request.param.endswith("push")
     if push:  # read when the context is created
@@ -59,6 +61,9 @@ def eng2(request):
         e.set_default_tuning({"schedule": "launch", "outer_split": 1, "outer_items": 2048, "graphs": False,
                               "intra_wide": 0 if narrow else 1 << 30, "outer_kgroup": 0 if narrow else 2,
                               "closure_ahead": ahead})
+    elif request.param == "pipe-t32":
+        e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": False,
+                              "intra_wide": 256, "outer_kgroup": 1, "closure_ahead": True, "tile": 32})
     elif sched == "pipe" and request.param != "pipe":
         e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": False,
                               "intra_wide": 0 if push else 256, "outer_kgroup": 1, "closure_ahead": ahead})
@@ -375,9 +380,11 @@ def test_autotuned_schedule_is_used_and_exact():
             Fo, so = O.bpmax(s1, s2, intra, inter)
             assert score == so and _bits_equal(F, Fo)
         for setting in [{"schedule": "launch", "outer_split": 1, "outer_items": 64, "graphs": False,
-                         "intra_wide": 0, "outer_kgroup": 0, "closure_ahead": True},
+                         "intra_wide": 0, "outer_kgroup": 0, "closure_ahead": True, "tile": 0},
                         {"schedule": "pipe", "outer_split": 0, "outer_items": 4096, "graphs": True,
-                         "intra_wide": 1 << 30, "outer_kgroup": 3, "closure_ahead": False}]:
+                         "intra_wide": 1 << 30, "outer_kgroup": 3, "closure_ahead": False, "tile": 32},
+                        {"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": False,
+                         "intra_wide": 256, "outer_kgroup": 1, "closure_ahead": True, "tile": 32}]:
             eng.set_tuning(6, 70, 1, setting)
             assert eng.get_tuning(6, 70, 1) == setting
             F, score = eng.solve(s1, s2, intra, inter)
