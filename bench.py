@@ -260,7 +260,18 @@ def r0_leg(B, eng, n=512, batch=4096, reps=3):
         eng.tri_fill_random(A, n, batch, 1, 0, 200)
         eng.tri_fill_random(Bm, n, batch, 2, 0, 200)
         eng.tri_fill_random(C, n, batch, 3, 0, 200)
+        # two output matrices checked against a numpy max-plus of their operands
+        picks = [0, batch - 1]
+        ops = {m: [eng.tri_download(d + 4 * m * tri, n, 1)[0] for d in (A, Bm, C)] for m in picks}
         eng.r0_tri_batch(n, batch, A, Bm, C)
+        checked = True
+        for m in picks:
+            a, b, c = ops[m]
+            want = c.copy()
+            for i in range(n):
+                want[i, i:] = np.maximum(want[i, i:], (a[i, i:, None] + b[i:, i:]).max(axis=0))
+            got = eng.tri_download(C + 4 * m * tri, n, 1)[0]
+            checked = checked and bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
         eng.set_timing(True)
         eng.reset_stats()
         for _ in range(reps):
@@ -273,7 +284,8 @@ def r0_leg(B, eng, n=512, batch=4096, reps=3):
         log(f"r0 isolated {avg_ms:.2f} ms/launch, {tf:.2f} TFLOP/s")
         return {"workload": f"{batch} upper-triangular {n}x{n} fp32 triples, integers U[0,200], -inf below",
                 "ms_per_launch": round(avg_ms, 3), "achieved": round(tf, 3), "unit": "TFLOP/s",
-                "frac": round(tf / FP32_VALU_PEAK_TFLOPS, 4), "kernel": "k_r0tri<64>"}
+                "frac": round(tf / FP32_VALU_PEAK_TFLOPS, 4), "kernel": "k_r0tri<64>",
+                "checked": {"matrices": picks, "equal_to_numpy_maxplus": checked}}
     finally:
         for p in (A, Bm, C):
             eng.tri_free(p)

@@ -214,6 +214,41 @@ def test_r0_isolated_vs_oracle(eng, n, batch):
         assert np.all(got[m][~up] == -np.inf)
 
 
+def _maxplus_tri_np(A, Bm, C):
+    """C max= A (max,+) B on one upper-triangular pair, row by row in numpy (k in [i, j]:
+    B's -inf lower half masks k > j, A's masks k < i)."""
+    n = A.shape[0]
+    out = C.copy()
+    for i in range(n):
+        prod = (A[i, i:, None] + Bm[i:, i:]).max(axis=0)
+        out[i, i:] = np.maximum(out[i, i:], prod)
+    return out
+
+
+def test_r0_isolated_at_full_batch_size(eng):
+    """The isolated R0 kernel at its benchmark size, 4096 tiled 512x512 triangles of
+    synthetic integers (device-generated hash values, checked against their host formula),
+    with four whole output matrices checked against a numpy max-plus."""
+    n, batch = 512, 4096
+    _say(f"r0 isolated n={n} batch={batch}")
+    tri = eng.tri_tiled_floats(n)
+    dA, dB, dC = eng.tri_alloc(n, batch), eng.tri_alloc(n, batch), eng.tri_alloc(n, batch)
+    try:
+        for d, seed in ((dA, 1), (dB, 2), (dC, 3)):
+            eng.tri_fill_random(d, n, batch, seed, 0, 200)
+        picks = [0, 1, 2047, 4095]
+        before = {m: [eng.tri_download(d + 4 * m * tri, n, 1)[0] for d in (dA, dB, dC)] for m in picks}
+        eng.r0_tri_batch(n, batch, dA, dB, dC)
+        for m in picks:
+            A, Bm, C = before[m]
+            want = _maxplus_tri_np(A, Bm, C)
+            got = eng.tri_download(dC + 4 * m * tri, n, 1)[0]
+            assert _bits_equal(got, want), m
+    finally:
+        for p in (dA, dB, dC):
+            eng.tri_free(p)
+
+
 def _tree6(M, N, N_tile):
     """The reference driver's 6-level pointer tree over one contiguous buffer
     (src/bpmax-wrapper.c:175-197), built with ctypes."""
