@@ -218,8 +218,22 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
                                  "note": "one-stream schedule, one R0 launch per outer diagonal, no overlap"}
     log(f"k_outer {achieved:.2f} TFLOP/s in the pipelined run, {r0_serial:.2f} alone; phases {serial}")
     eng.set_tuning(M, N, 1, tuned)  # the one-stream breakdown above overrode the class
+    # the dependency-tile scheduler (one persistent launch per solve) beside the graph path
+    eng.set_schedule("tasks", 64)
+    eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    t0 = time.perf_counter()
+    _, sc_t = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
+    ms_tasks = (time.perf_counter() - t0) * 1e3
+    ts = eng.task_stats()
+    eng.set_schedule(B.SCHED_PIPE)
+    tasks = {"ms": round(ms_tasks, 2), "score_equal": sc_t == score, "chunk": 64, "tasks": ts["tasks"],
+             "wait_polls": ts["wait_polls"],
+             "note": "one persistent launch; workgroups take tasks in a topological order and wait on "
+                     "per-tile counters (agent-scope acquire/release per task)"}
+    log(f"dependency-tile scheduler: {ms_tasks:.1f} ms ({ts['tasks']} tasks, {ts['wait_polls']:.3g} waiting polls)")
     return {"score": score, "ms_per_step": ms_per_step, "ms_plain": ms_plain, "gcups": gcups,
-            "roofline": roofline, "phases_pipelined": phases, "phases_one_stream": serial, "tuning": tuned}
+            "roofline": roofline, "phases_pipelined": phases, "phases_one_stream": serial, "tuning": tuned,
+            "dependency_tiles": tasks}
 
 
 def batched_leg(B, eng, reps=2):
@@ -461,6 +475,7 @@ def main():
             "roofline": res["roofline"], "cpu_baseline": cpu,
             "ms_per_step_without_events": round(res["ms_plain"], 3),
             "phases_pipelined": res["phases_pipelined"], "phases_one_stream": res["phases_one_stream"],
+            "dependency_tiles": res["dependency_tiles"],
             "batched": batched, "r0_isolated": r0, "serving": serving}
     print(json.dumps(line), flush=True)
     eng.close()

@@ -78,6 +78,7 @@ def load_library(path: str = LIB_PATH):
         "bpmax_ctx_kernel_stats": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_longlong),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "bpmax_ctx_reset_stats": (c_int, [c_void_p]),
+        "bpmax_ctx_task_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]),
         "bpmax_ctx_set_schedule": (c_int, [c_void_p, c_int, c_int]),
         "bpmax_ctx_set_graphs": (c_int, [c_void_p, c_int]),
         "bpmax_ctx_stream": (c_void_p, [c_void_p]),
@@ -366,6 +367,13 @@ class Engine:
             for c in classes:
                 mask |= 1 << KERNEL_CLASSES.index(c)
             self._check(self.L.bpmax_ctx_set_timing_mask(self.h, mask))
+
+    def task_stats(self) -> dict:
+        """Tasks run and polls spent waiting in the last task-scheduler run."""
+        n = ctypes.c_longlong()
+        p = ctypes.c_double()
+        self._check(self.L.bpmax_ctx_task_stats(self.h, ctypes.byref(n), ctypes.byref(p)))
+        return {"tasks": n.value, "wait_polls": p.value}
 
     def reset_stats(self):
         self._check(self.L.bpmax_ctx_reset_stats(self.h))

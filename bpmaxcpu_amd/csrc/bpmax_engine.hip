@@ -426,6 +426,14 @@ extern "C" int bpmax_tune_shape(bpmax_ctx *ctx, int M, int N, int npairs, int re
     return tune_pairs(ctx, npairs, p1.data(), Ms.data(), p2.data(), Ns.data(), reps, best_out, best_ms);
 }
 
+extern "C" int bpmax_ctx_task_stats(bpmax_ctx *ctx, long long *tasks, double *wait_polls)
+{
+    if (!ctx) return BPMAX_EINVAL;
+    if (tasks) *tasks = ctx->task_count;
+    if (wait_polls) *wait_polls = (double)ctx->task_polls;
+    return BPMAX_OK;
+}
+
 extern "C" int bpmax_ctx_reset_stats(bpmax_ctx *ctx)
 {
     if (!ctx) return BPMAX_EINVAL;
@@ -915,11 +923,25 @@ int bpx_collect_timing(bpmax_ctx *ctx)
     return BPMAX_OK;
 }
 
-// One phase launch of the schedule on stream st (nothing for an empty position).
+// One phase launch of the schedule on stream st (nothing for an empty position).  A launch
+// of more than kMaxItems work items (one workgroup each) goes out in pieces: the items of
+// one launch are independent, and a grid stays far below the 2^32 threads a dispatch holds.
+constexpr int kMaxItems = 1 << 22;
 static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDesc *dP, const WorkItem *dI,
                       float *arena, const int *dcodes, const float *dw)
 {
     if (L.count == 0) return BPMAX_OK;
+    if (L.count > kMaxItems) {
+        for (int f = 0; f < L.count; f += kMaxItems) {
+            Launch part = L;
+            part.first = L.first + f;
+            part.count = std::min(kMaxItems, L.count - f);
+            part.updates = f == 0 ? L.updates : 0;
+            const int rc = run_launch(ctx, st, part, dP, dI, arena, dcodes, dw);
+            if (rc != BPMAX_OK) return rc;
+        }
+        return BPMAX_OK;
+    }
     const WorkItem *items = dI + L.first;
     switch (L.kind) {
     case BPMAX_K_OUTER:
@@ -1548,9 +1570,14 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
             });
             if (rc != BPMAX_OK) return rc;
         }
-        int err = 0;
-        HIPCHK(ctx, hipMemcpyAsync(&err, dflags + 8, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        int ew[6] = {0, 0, 0, 0, 0, 0}; // error word, pad, 64-bit poll count, tasks run, pad
+        HIPCHK(ctx, hipMemcpyAsync(ew, dflags + 8, sizeof ew, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        const int err = ew[0];
+        unsigned long long polls = 0;
+        memcpy(&polls, ew + 2, sizeof polls);
+        ctx->task_polls = polls;
+        ctx->task_count = ew[4];
         if (err) {
             ctx->err = "task scheduler: a dependency wait timed out (kernel abandoned its tasks)";
             return BPMAX_EDEVICE;

@@ -89,6 +89,8 @@ struct bpmax_ctx {
     std::vector<int> flags_host, flags_staging;
     int schedule = BPMAX_SCHED_PIPE;
     int task_chunk = 16;
+    unsigned long long task_polls = 0;   // last task-scheduler run: waiting polls (sum over tasks
+    long long task_count = 0;            // of the largest lane count) and tasks run
     std::vector<bpx::PairDesc> pairs_host;
     bool result_valid = false;           // pairs_host's F tables are resident in the arena
     bpx::Schedule sched;

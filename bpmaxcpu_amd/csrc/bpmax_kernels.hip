@@ -712,9 +712,10 @@ __global__ __launch_bounds__(kThreads) void k_s1(const PairDesc *__restrict__ pa
 // ------------------------------------------------------- dependency-tile scheduler
 // Every task of a solve (S1, S2 tiles, then per outer diagonal the R0 chunks of every
 // storage tile and the cross-tile chunks and in-tile closures of every logical tile) is
-// one entry of a list in a topological order.  Workgroups take tickets from an atomic
-// head, so a task only ever waits on tasks holding smaller tickets, which are already
-// running or finished: no deadlock, whatever the grid size.  Each logical F / S2 tile has
+// one entry of a list in a topological order.  A fixed grid of persistent workgroups takes
+// tickets from an atomic head, one task at a time, so a task only ever waits on tasks
+// holding smaller tickets, which are already running or finished: no deadlock, whatever the
+// grid size, and every workgroup leaves once the tickets run out.  Each logical F / S2 tile has
 // two counters in `flags`:
 //   done    = 1 once its in-tile closure has stored the final values;
 //   pending = number of max-combining contributions (R0 chunks of its storage tile,
@@ -754,82 +755,134 @@ __device__ __forceinline__ int *s_ctr(int *flags, const PairDesc &P, int X, int 
     return flags + P.flag + 2 + 2 * utri_ll(X, Y, P.nbl);
 }
 
-// wait until every upper logical sub-tile of storage tile (SI,SJ) of triangle (i1,j1) is done
-__device__ __forceinline__ void wait_storage(Waiter &W, int *flags, const PairDesc &P, int i1, int j1, int SI, int SJ)
+// Dependencies of a task, enumerated so that every lane of the workgroup can poll its share
+// of them at once: dependency i is a counter and the value it must reach (want); entries
+// that do not apply come back with p = nullptr.  The set is, per task type:
+//   S2 cross-tile chunk: per step s (K = I+s) the tiles (I,K), (K,J) (K > I) and (K+1,J) done;
+//   S2 in-tile: its pending contributions landed; for I < J the tiles (I,I), (J,J), (I+1,J),
+//     (I,J-1) and, for J-I >= 2, (I+1,J-1) done;
+//   R0 chunk: S1 done, every S2 tile done, then per step (k1, K) the logical sub-tiles of
+//     storage tiles (I,K) of F(i1,k1) and (K,J), (K+1,J) of F(k1+1,j1); with the epilogue
+//     also storage tile (I,J) of F(i1+1,j1-1) (P1, d1 >= 4) and of F(k1+1,j1), F(i1,k1) for
+//     every k1 (R3/R4);
+//   cross-tile chunk: every S2 tile done, then per step the one tile it reads;
+//   in-tile: every S2 tile done, its pending contributions landed, and for I < J the tiles
+//     (I+1,J), (I,J-1) and, for J-I >= 2, (I+1,J-1).
+struct Dep {
+    const int *p;
+    int want;
+};
+
+// logical sub-tile q (0 .. rs*rs-1) of storage tile (SI,SJ) of triangle (i1,j1); the lower
+// sub-tiles of a diagonal storage tile (x > y) do not exist
+__device__ __forceinline__ Dep storage_dep(int *flags, const PairDesc &P, int rs, int i1, int j1, int SI, int SJ,
+                                           int q)
 {
-    const int rs = P.nbl / P.nb;
-    for (int x = SI * rs; x < SI * rs + rs; ++x)
-        for (int y = SJ * rs; y < SJ * rs + rs; ++y)
-            if (x <= y) W.until(f_ctr(flags, P, i1, j1, x, y), 1);
+    const int x = SI * rs + q / rs, y = SJ * rs + q % rs;
+    if (x > y) return Dep{nullptr, 0};
+    return Dep{f_ctr(flags, P, i1, j1, x, y), 1};
 }
 
-// Thread 0 only: wait for everything task `it` reads.  Returns false on timeout/abort.
-__device__ bool wait_deps(const PairDesc &P, const WorkItem &it, int *flags, const int *err)
+__device__ int dep_count(const PairDesc &P, const WorkItem &it)
 {
-    Waiter W{err};
-    const int type = wi_type(it.flags);
-    const int I = it.I, J = it.J;
-    switch (type) {
-    case TASK_S1:
-        break;
-    case TASK_S2EXT:
-        for (int s = it.step_begin; s < it.step_end; ++s) {
-            const int K = I + s;
-            W.until(s_ctr(flags, P, I, K), 1);
-            if (K > I) W.until(s_ctr(flags, P, K, J), 1);
-            W.until(s_ctr(flags, P, K + 1, J), 1);
+    const int I = it.I, J = it.J, ns = it.step_end - it.step_begin, rs = P.nbl / P.nb;
+    switch (wi_type(it.flags)) {
+    case TASK_S2EXT: return 3 * ns;
+    case TASK_S2INTRA: return 6;
+    case TASK_OUTER: {
+        int n = 2 + 3 * rs * rs * ns;
+        if (it.flags & WI_EPILOGUE) n += rs * rs * (1 + 2 * (it.j1 - it.i1));
+        return n;
+    }
+    case TASK_EXT: return 1 + ns;
+    case TASK_INTRA: return 5;
+    default: return 0;
+    }
+    (void)I;
+    (void)J;
+}
+
+__device__ Dep dep_at(const PairDesc &P, const WorkItem &it, int *flags, int i)
+{
+    const int I = it.I, J = it.J, rs = P.nbl / P.nb;
+    const Dep none{nullptr, 0};
+    const Dep s2_all{flags + P.flag + 1, (int)tri_ll(P.nbl)};
+    switch (wi_type(it.flags)) {
+    case TASK_S2EXT: {
+        const int s = it.step_begin + i / 3, K = I + s;
+        switch (i % 3) {
+        case 0: return Dep{s_ctr(flags, P, I, K), 1};
+        case 1: return K > I ? Dep{s_ctr(flags, P, K, J), 1} : none;
+        default: return Dep{s_ctr(flags, P, K + 1, J), 1};
         }
-        break;
+    }
     case TASK_S2INTRA:
-        W.until(s_ctr(flags, P, I, J) + 1, 0);
-        if (I < J) {
-            W.until(s_ctr(flags, P, I, I), 1);
-            W.until(s_ctr(flags, P, J, J), 1);
-            W.until(s_ctr(flags, P, I + 1, J), 1);
-            W.until(s_ctr(flags, P, I, J - 1), 1);
-            if (J - I >= 2) W.until(s_ctr(flags, P, I + 1, J - 1), 1);
+        if (i == 0) return Dep{s_ctr(flags, P, I, J) + 1, 0};
+        if (I == J) return none;
+        switch (i) {
+        case 1: return Dep{s_ctr(flags, P, I, I), 1};
+        case 2: return Dep{s_ctr(flags, P, J, J), 1};
+        case 3: return Dep{s_ctr(flags, P, I + 1, J), 1};
+        case 4: return Dep{s_ctr(flags, P, I, J - 1), 1};
+        default: return J - I >= 2 ? Dep{s_ctr(flags, P, I + 1, J - 1), 1} : none;
         }
-        break;
     case TASK_OUTER: { // storage coordinates
-        const int i1 = it.i1, j1 = it.j1, nK = J - I + 1;
-        W.until(flags + P.flag, 1);                      // S1
-        W.until(flags + P.flag + 1, (int)tri_ll(P.nbl)); // every S2 tile
-        for (int s = it.step_begin; s < it.step_end; ++s) {
+        const int i1 = it.i1, j1 = it.j1, nK = J - I + 1, q2 = rs * rs;
+        if (i == 0) return Dep{flags + P.flag, 1}; // S1
+        if (i == 1) return s2_all;
+        i -= 2;
+        const int nst = 3 * q2 * (it.step_end - it.step_begin);
+        if (i < nst) {
+            const int s = it.step_begin + i / (3 * q2), r = i % (3 * q2), g = r / q2, q = r % q2;
             const int k1 = i1 + s / nK, K = I + s % nK;
-            wait_storage(W, flags, P, i1, k1, I, K);
-            wait_storage(W, flags, P, k1 + 1, j1, K, J);
-            if (K + 1 <= J) wait_storage(W, flags, P, k1 + 1, j1, K + 1, J);
+            if (g == 0) return storage_dep(flags, P, rs, i1, k1, I, K, q);
+            if (g == 1) return storage_dep(flags, P, rs, k1 + 1, j1, K, J, q);
+            return K + 1 <= J ? storage_dep(flags, P, rs, k1 + 1, j1, K + 1, J, q) : none;
         }
-        if (it.flags & WI_EPILOGUE) {
-            if (j1 - i1 >= 4) wait_storage(W, flags, P, i1 + 1, j1 - 1, I, J);
-            for (int k1 = i1; k1 < j1; ++k1) {
-                wait_storage(W, flags, P, k1 + 1, j1, I, J);
-                wait_storage(W, flags, P, i1, k1, I, J);
-            }
-        }
-        break;
+        i -= nst; // epilogue
+        const int g = i / q2, q = i % q2;
+        if (g == 0) return j1 - i1 >= 4 ? storage_dep(flags, P, rs, i1 + 1, j1 - 1, I, J, q) : none;
+        const int k1 = i1 + (g - 1) / 2;
+        return (g - 1) % 2 == 0 ? storage_dep(flags, P, rs, k1 + 1, j1, I, J, q)
+                                : storage_dep(flags, P, rs, i1, k1, I, J, q);
     }
     case TASK_EXT: { // logical coordinates
-        const int D = J - I;
-        W.until(flags + P.flag + 1, (int)tri_ll(P.nbl));
-        for (int s = it.step_begin; s < it.step_end; ++s) {
-            if (s < D) W.until(f_ctr(flags, P, it.i1, it.j1, I, I + s), 1);
-            else W.until(f_ctr(flags, P, it.i1, it.j1, I + 1 + (s - D), J), 1);
-        }
-        break;
+        if (i == 0) return s2_all;
+        const int s = it.step_begin + i - 1, D = J - I;
+        return s < D ? Dep{f_ctr(flags, P, it.i1, it.j1, I, I + s), 1}
+                     : Dep{f_ctr(flags, P, it.i1, it.j1, I + 1 + (s - D), J), 1};
     }
     case TASK_INTRA:
-        W.until(flags + P.flag + 1, (int)tri_ll(P.nbl));
-        W.until(f_ctr(flags, P, it.i1, it.j1, I, J) + 1, 0);
-        if (I < J) {
-            W.until(f_ctr(flags, P, it.i1, it.j1, I + 1, J), 1);
-            W.until(f_ctr(flags, P, it.i1, it.j1, I, J - 1), 1);
-            if (J - I >= 2) W.until(f_ctr(flags, P, it.i1, it.j1, I + 1, J - 1), 1);
-        }
-        break;
+        if (i == 0) return s2_all;
+        if (i == 1) return Dep{f_ctr(flags, P, it.i1, it.j1, I, J) + 1, 0};
+        if (I == J) return none;
+        if (i == 2) return Dep{f_ctr(flags, P, it.i1, it.j1, I + 1, J), 1};
+        if (i == 3) return Dep{f_ctr(flags, P, it.i1, it.j1, I, J - 1), 1};
+        return J - I >= 2 ? Dep{f_ctr(flags, P, it.i1, it.j1, I + 1, J - 1), 1} : none;
+    default: return none;
     }
-    if (!W.ok) atomicOr((int *)err, 1);
-    return W.ok;
+}
+
+// Every lane of the workgroup: wait for everything task `it` reads, the dependencies spread
+// over the lanes (one round of polls instead of a chain of them on one thread).  Each lane's
+// agent-scope acquire after its polls pairs with the producers' releases, and the barrier
+// that follows carries that to every lane.  Returns false on a timeout or when another task
+// gave up (*err set).
+// *spins (shared) receives the largest number of unsatisfied polls of any lane: the task's
+// time spent waiting, in polls (each a device-scope load plus s_sleep 2).
+__device__ bool wait_deps(const PairDesc &P, const WorkItem &it, int *flags, const int *err, int *spins)
+{
+    const int n = dep_count(P, it);
+    Waiter W{err};
+    for (int i = threadIdx.x; i < n && W.ok; i += blockDim.x) {
+        const Dep d = dep_at(P, it, flags, i);
+        if (d.p) W.until(d.p, d.want);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (W.polls) atomicMax(spins, W.polls);
+    const bool ok = __syncthreads_and(W.ok ? 1 : 0) != 0;
+    if (!ok && threadIdx.x == 0) atomicOr((int *)err, 1);
+    return ok;
 }
 
 // Thread 0 only, after every lane's stores were released.
@@ -875,30 +928,43 @@ __global__ __launch_bounds__(kThreads) void k_tasks(const PairDesc *__restrict__
     constexpr int LT = Inner<T>::LT;
     constexpr int SM = cmax(OuterLds<T>::FLOATS, cmax(Tc<LT>::PROD_FLOATS, Intra<LT>::FLOATS));
     __shared__ __attribute__((aligned(16))) float smem[SM];
-    __shared__ int s_task, s_ok;
-    if (threadIdx.x == 0) s_task = atomicAdd(head, 1);
-    __syncthreads();
-    const int t = s_task;
-    if (t >= ntasks) return;
-    const WorkItem it = tasks[t];
-    const PairDesc P = pairs[it.pair];
-    if (threadIdx.x == 0) {
-        s_ok = wait_deps(P, it, flags, err);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __shared__ int s_task, s_spins;
+    // persistent: every workgroup takes tickets until the list is exhausted (or a task gave
+    // up, *err), so the grid is bounded whatever the number of tasks.  err[2..3] count, as a
+    // 64-bit sum, the polls workgroups spent waiting (the largest lane count per task);
+    // err[4] counts tasks run.
+    unsigned long long waited = 0;
+    int ran = 0;
+    for (;;) {
+        __syncthreads(); // s_task and smem of the previous task are free
+        if (threadIdx.x == 0) {
+            s_task = ld_relaxed(err) != 0 ? ntasks : atomicAdd(head, 1);
+            s_spins = 0;
+        }
+        __syncthreads();
+        const int t = s_task;
+        if (t >= ntasks) break;
+        const WorkItem it = tasks[t];
+        const PairDesc P = pairs[it.pair];
+        if (!wait_deps(P, it, flags, err, &s_spins)) break;
+        waited += (unsigned)s_spins;
+        ++ran;
+        switch (wi_type(it.flags)) {
+        case TASK_S1: s1_body(P, arena, codes, w); break;
+        case TASK_S2EXT: ext_body<T, true>(P, it, arena, smem); break;
+        case TASK_S2INTRA: intra_body<T, true>(P, 0, 0, it.I, it.J, arena, codes, w, smem); break;
+        case TASK_OUTER: outer_body<T, false>(P, it, arena, codes, w, smem); break;
+        case TASK_EXT: ext_body<T, false>(P, it, arena, smem); break;
+        case TASK_INTRA: intra_body<T, false>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem); break;
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) signal_done(P, it, flags);
     }
-    __syncthreads();
-    if (!s_ok) return;
-    switch (wi_type(it.flags)) {
-    case TASK_S1: s1_body(P, arena, codes, w); break;
-    case TASK_S2EXT: ext_body<T, true>(P, it, arena, smem); break;
-    case TASK_S2INTRA: intra_body<T, true>(P, 0, 0, it.I, it.J, arena, codes, w, smem); break;
-    case TASK_OUTER: outer_body<T, false>(P, it, arena, codes, w, smem); break;
-    case TASK_EXT: ext_body<T, false>(P, it, arena, smem); break;
-    case TASK_INTRA: intra_body<T, false>(P, it.i1, it.j1, it.I, it.J, arena, codes, w, smem); break;
+    if (threadIdx.x == 0 && ran) {
+        atomicAdd((unsigned long long *)(err + 2), waited);
+        atomicAdd(err + 4, ran);
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) signal_done(P, it, flags);
 }
 
 // ------------------------------------------------------------------------- utilities
@@ -1125,7 +1191,10 @@ hipError_t launch_tasks(hipStream_t st, int T, const PairDesc *pairs, const Work
                         int *flags, int *err, float *arena, const int *codes, const float *w)
 {
     if (ntasks <= 0) return hipSuccess;
-    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_tasks<TT>, dim3(ntasks), dim3(kThreads), 0, st, pairs, tasks, ntasks,
+    // persistent workgroups: twice what fits on the device at once (4 per CU), enough to keep
+    // every slot busy while the last tasks drain
+    const int grid = ntasks < 2048 ? ntasks : 2048;
+    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_tasks<TT>, dim3(grid), dim3(kThreads), 0, st, pairs, tasks, ntasks,
                                          head, flags, err, arena, codes, w));
     return hipGetLastError();
 }

@@ -243,6 +243,10 @@ typedef enum {
 int bpmax_ctx_kernel_stats(bpmax_ctx *ctx, int kclass, long long *launches, double *ms,
                            double *updates);
 int bpmax_ctx_reset_stats(bpmax_ctx *ctx);
+/* Last BPMAX_SCHED_TASKS run: tasks run, and the polls its workgroups spent waiting for
+ * dependencies (per task the largest count of any lane, summed; a poll is one device-scope
+ * load plus a short sleep). */
+int bpmax_ctx_task_stats(bpmax_ctx *ctx, long long *tasks, double *wait_polls);
 /* Schedules:
  *   BPMAX_SCHED_PIPE (default): one launch per phase and block diagonal on two streams --
  *     R0 work of outer diagonal d1 on one, the inner closure of d1 on the other, joined by
