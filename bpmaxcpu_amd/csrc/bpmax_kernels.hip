@@ -230,19 +230,58 @@ __device__ __forceinline__ ExtStep ext_step(int I, int J, int s)
     return ExtStep{K2, -1, K2, 0, false, true, false, false, K2 == J};
 }
 
-// Steps [sb, se) of the cross-tile part of logical tile (I,J) of X on 256 lanes, folded
-// into acc (this lane's R x R block at lane_block).
+// Cross-tile products split k over the four waves: on every step, wave w takes k in
+// [w*LT/4, (w+1)*LT/4) on the whole LT x LT tile, and each lane keeps a 4 x 4 block of cells
+// (lane l: rows 4*(l / (LT/4)), columns 4*(l % (LT/4))).  A lane then reads 8 operand floats
+// per k for 16 cell updates (a 2 x 2 block over all k would read 4 for 4), half the LDS bytes
+// per update, for the same VALU work.  The four partial maxima meet once per item, in LDS.
+template <int LT>
+struct Xk {
+    static_assert(LT == 32, "cross-tile k split: 64 lanes of 4 x 4 blocks per 32 x 32 tile");
+    static constexpr int KQ = LT / 4;                 // k per wave per step
+    static constexpr int LD = Tc<LT>::LD;             // row stride of the partials in LDS
+    static constexpr int RED_FLOATS = 4 * LT * LD;    // one partial tile per wave
+};
+
+// acc[i][j] = max(acc[i][j], max_{kb<=k<ke} As[k][r0+i] + Bs[k][c0+j]), kb and ke even: two k
+// per round, v_add_f32 x 2 + v_max3_f32 per pair of updates.
+template <int LT>
+__device__ __forceinline__ void product44(float (&acc)[4][4], const float *As, const float *Bs, int r0, int c0,
+                                          int kb, int ke)
+{
+    constexpr int LD = Tc<LT>::LD;
+#pragma unroll 1 // unrolled, the loads of later rounds are hoisted past 64 VGPRs
+    for (int k = kb; k < ke; k += 2) {
+        const float4 a0 = *(const float4 *)(As + k * LD + r0), a1 = *(const float4 *)(As + (k + 1) * LD + r0);
+        const float4 b0 = *(const float4 *)(Bs + k * LD + c0), b1 = *(const float4 *)(Bs + (k + 1) * LD + c0);
+        const float av0[4] = {a0.x, a0.y, a0.z, a0.w}, av1[4] = {a1.x, a1.y, a1.z, a1.w};
+        const float bv0[4] = {b0.x, b0.y, b0.z, b0.w}, bv1[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = max3(acc[i][j], av0[i] + bv0[j], av1[i] + bv1[j]);
+    }
+}
+
+// Steps [sb, se) of the cross-tile part of logical tile (I,J) of X on 256 lanes.  On return
+// `out` holds the maxima of the four cells (tid / (LT/4), 4*(tid % (LT/4)) + 0..3) of the
+// tile (-inf with no steps); smem is free again.
 template <int T, bool MODE_S>
-__device__ void ext_accumulate(const PairDesc &P, float *X, int I, int J, int sb, int se,
-                               float (&acc)[Tc<Inner<T>::LT>::R][Tc<Inner<T>::LT>::R], float *arena, float *smem)
+__device__ void ext_accumulate(const PairDesc &P, float *X, int I, int J, int sb, int se, float4 &out, float *arena,
+                               float *smem)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int LD = Tc<LT>::LD;
+    constexpr int LD = Tc<LT>::LD, KQ = Xk<LT>::KQ, BPR = LT / 4; // 4 x 4 blocks per row
     float *As = smem, *Bs = smem + LT * LD;
-    int r0, c0;
-    lane_block<LT>(r0, c0);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
+    const int r0 = 4 * (l / BPR), c0 = 4 * (l % BPR);
     const int nb = P.nb;
     float *S = arena + P.S2;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = NEG_INF;
     FetchA<LT> fa;
     FetchB<LT> fb;
     ExtStep st{};
@@ -256,12 +295,28 @@ __device__ void ext_accumulate(const PairDesc &P, float *X, int I, int J, int sb
         __syncthreads();
         commit_A<LT>(As, fa, st.cshift, st.lastcol);
         commit_B<LT>(Bs, fb);
-        int kb, ke;
-        quad_k_range<LT>(st.adiag, st.bdiag, st.lastcol, kb, ke);
+        // with only column LT-1 of A finite, the pair of k values (LT-2, LT-1) is the step
+        const int kb = st.lastcol ? (w == 3 ? LT - 2 : LT) : w * KQ, ke = st.lastcol ? LT : (w + 1) * KQ;
         __syncthreads();
         if (s + 1 < se) fetch(s + 1);
-        product<LT>(acc, As, Bs, r0, c0, kb, ke);
+        product44<LT>(acc, As, Bs, r0, c0, kb, ke);
     }
+    // the four partial tiles meet in LDS; lane tid takes four cells of one row
+    float *red = smem;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        *(float4 *)(red + w * LT * LD + (r0 + i) * LD + c0) = make_float4(acc[i][0], acc[i][1], acc[i][2], acc[i][3]);
+    __syncthreads();
+    const int row = threadIdx.x / BPR, col = 4 * (threadIdx.x % BPR);
+    float4 m = *(const float4 *)(red + row * LD + col);
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+        const float4 v = *(const float4 *)(red + q * LT * LD + row * LD + col);
+        m = make_float4(fmaxf(m.x, v.x), fmaxf(m.y, v.y), fmaxf(m.z, v.z), fmaxf(m.w, v.w));
+    }
+    out = m;
+    __syncthreads();
 }
 
 // Always max-combined into the tile (other items of the same tile run concurrently).
@@ -269,17 +324,15 @@ template <int T, bool MODE_S>
 __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int R = Tc<LT>::R;
     float *X = MODE_S ? arena + P.S2 : tri_at(arena, P, it.i1, it.j1);
-    float acc[R][R];
-#pragma unroll
-    for (int i = 0; i < R; ++i)
-#pragma unroll
-        for (int j = 0; j < R; ++j) acc[i][j] = NEG_INF;
-    ext_accumulate<T, MODE_S>(P, X, it.I, it.J, it.step_begin, it.step_end, acc, arena, smem);
-    int r0, c0;
-    lane_block<LT>(r0, c0);
-    store_acc<T, LT>(ltile<T, LT>(X, it.I, it.J, P.nb), acc, r0, c0, true);
+    float4 m;
+    ext_accumulate<T, MODE_S>(P, X, it.I, it.J, it.step_begin, it.step_end, m, arena, smem);
+    const int row = threadIdx.x / (LT / 4), col = 4 * (threadIdx.x % (LT / 4));
+    int *dst = (int *)(ltile<T, LT>(X, it.I, it.J, P.nb) + row * T + col);
+    atomicMax(dst + 0, __float_as_int(m.x));
+    atomicMax(dst + 1, __float_as_int(m.y));
+    atomicMax(dst + 2, __float_as_int(m.z));
+    atomicMax(dst + 3, __float_as_int(m.w));
 }
 
 // ---------------------------------------------------------------------- inner intra
@@ -336,18 +389,10 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     // late cross-tile steps: staged in the LDS that the set-up below overwrites, results
     // kept in registers until the tile is in LDS
     constexpr bool kLate = !MODE_S && BLK;
-    constexpr int RX = Tc<LT>::R;
-    float eacc[RX][RX];
+    float4 emax = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
     const bool late_steps = kLate && late && J > I;
     if constexpr (kLate) {
-        if (late_steps) {
-#pragma unroll
-            for (int i = 0; i < RX; ++i)
-#pragma unroll
-                for (int j = 0; j < RX; ++j) eacc[i][j] = NEG_INF;
-            ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, eacc, arena, smem);
-            __syncthreads();
-        }
+        if (late_steps) ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, emax, arena, smem);
     }
 
     // tiles: every global load issued before any LDS store (float4, unrolled, branch-free;
@@ -411,16 +456,12 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     if (tid < 25) wl[tid] = vw;
     if constexpr (kLate) {
         if (late_steps) { // fold the late steps into the tile (each cell has one owner)
-            int er0, ec0;
-            lane_block<LT>(er0, ec0);
             __syncthreads();
-#pragma unroll
-            for (int i = 0; i < RX; ++i)
-#pragma unroll
-                for (int j = 0; j < RX; ++j) {
-                    float *c = Xs + (er0 + i) * LX + ec0 + j;
-                    *c = fmaxf(*c, eacc[i][j]);
-                }
+            float *c = Xs + (tid / (LT / 4)) * LX + 4 * (tid % (LT / 4));
+            c[0] = fmaxf(c[0], emax.x);
+            c[1] = fmaxf(c[1], emax.y);
+            c[2] = fmaxf(c[2], emax.z);
+            c[3] = fmaxf(c[3], emax.w);
         }
     }
     // SL = S2 (I,I), SR = S2 (J,J) (both in Sb on a diagonal tile); S2's own diagonal tiles
@@ -642,7 +683,7 @@ template <int T, bool MODE_S>
 __global__ __launch_bounds__(kThreads, 8) void k_inner_ext(const PairDesc *__restrict__ pairs,
                                                         const WorkItem *__restrict__ items, float *__restrict__ arena)
 {
-    __shared__ __attribute__((aligned(16))) float smem[Tc<Inner<T>::LT>::PROD_FLOATS];
+    __shared__ __attribute__((aligned(16))) float smem[Xk<Inner<T>::LT>::RED_FLOATS];
     __builtin_amdgcn_s_setprio(3); // latency-critical chain: issue ahead of R0 waves
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     ext_body<T, MODE_S>(pairs[it.pair], it, arena, smem);
@@ -693,7 +734,7 @@ __global__ __launch_bounds__(kThreads, 8) void k_inner_closure_blk(const PairDes
                                                                 const int *__restrict__ codes,
                                                                 const float *__restrict__ w)
 {
-    static_assert(Tc<Inner<T>::LT>::PROD_FLOATS <= Intra<Inner<T>::LT>::FLOATS, "staging fits the in-tile LDS");
+    static_assert(Xk<Inner<T>::LT>::RED_FLOATS <= Intra<Inner<T>::LT>::FLOATS, "cross-tile LDS fits the in-tile LDS");
     __shared__ __attribute__((aligned(16))) float smem[Intra<Inner<T>::LT>::FLOATS];
     __builtin_amdgcn_s_setprio(3);
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
@@ -926,7 +967,7 @@ __global__ __launch_bounds__(kThreads) void k_tasks(const PairDesc *__restrict__
                                                     const int *__restrict__ codes, const float *__restrict__ w)
 {
     constexpr int LT = Inner<T>::LT;
-    constexpr int SM = cmax(OuterLds<T>::FLOATS, cmax(Tc<LT>::PROD_FLOATS, Intra<LT>::FLOATS));
+    constexpr int SM = cmax(OuterLds<T>::FLOATS, cmax(Xk<LT>::RED_FLOATS, Intra<LT>::FLOATS));
     __shared__ __attribute__((aligned(16))) float smem[SM];
     __shared__ int s_task, s_spins;
     // persistent: every workgroup takes tickets until the list is exhausted (or a task gave
