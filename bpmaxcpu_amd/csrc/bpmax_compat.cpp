@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -64,9 +65,10 @@ extern "C" void bpmax(long M, long N, long N_sec, long N_tile, long R, long MR, 
             exit(-1);
         }
     }
-    std::vector<float> packed((size_t)(M * (M + 1) / 2) * (size_t)(N * (N + 1) / 2));
+    // left uninitialised: the engine writes every element (zeroing 1 GB would cost ~0.1 s)
+    std::unique_ptr<float[]> packed(new float[(size_t)(M * (M + 1) / 2) * (size_t)(N * (N + 1) / 2)]);
     float score = 0.f;
-    int rc = bpmax_engine_solve(g_ctx, seq1, (int)M, seq2, (int)N, intra, inter, packed.data(),
+    int rc = bpmax_engine_solve(g_ctx, seq1, (int)M, seq2, (int)N, intra, inter, packed.get(),
                                 BPMAX_LAYOUT_PACKED_TRI_TRI, &score);
     if (rc != BPMAX_OK) {
         printf("bpmax: engine error %d: %s\n", rc, bpmax_last_error(g_ctx));
@@ -76,9 +78,13 @@ extern "C" void bpmax(long M, long N, long N_sec, long N_tile, long R, long MR, 
     // FTable[i1][j1][(i2+R)/N_tile][(j2+R)/N_tile][(i2+R)%N_tile][(j2+R)%N_tile]
     // (src/bpmax-wrapper.c:44)
     const size_t TN = (size_t)(N * (N + 1) / 2);
-    auto scatter = [&](long i1) {
-        for (long j1 = i1; j1 < M; ++j1) {
-            const float *tri = packed.data() + (size_t)(i1 * M - i1 * (i1 - 1) / 2 + (j1 - i1)) * TN;
+    std::vector<std::pair<long, long>> tris; // (i1, j1) in packed order: balanced work units
+    for (long i1 = 0; i1 < M; ++i1)
+        for (long j1 = i1; j1 < M; ++j1) tris.emplace_back(i1, j1);
+    auto scatter = [&](long u) {
+        const long i1 = tris[u].first, j1 = tris[u].second;
+        {
+            const float *tri = packed.get() + (size_t)u * TN;
             for (long i2 = 0; i2 < N; ++i2) {
                 const float *row = tri + (size_t)(i2 * N - i2 * (i2 - 1) / 2) - i2;
                 const long ri = i2 + R;
@@ -97,7 +103,7 @@ extern "C" void bpmax(long M, long N, long N_sec, long N_tile, long R, long MR, 
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; ++t)
         pool.emplace_back([&] {
-            for (long i1; (i1 = next.fetch_add(1)) < M;) scatter(i1);
+            for (long u; (u = next.fetch_add(1)) < (long)tris.size();) scatter(u);
         });
     for (auto &th : pool) th.join();
 }

@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -176,6 +177,10 @@ extern "C" int bpmax_ctx_destroy(bpmax_ctx *ctx)
     for (auto &e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (auto &e : ctx->sync_ev) (void)hipEventDestroy(e);
     for (void *p : ctx->user_allocs) (void)hipFree(p);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->pin[b]) (void)hipHostFree(ctx->pin[b]);
+        if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
+    }
     (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1691,9 +1696,63 @@ int bpx_copy_out(bpmax_ctx *ctx, const float *arena, const PairDesc &P, bpmax_la
         });
     }
     if (rc != BPMAX_OK) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(F_out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = bpx_d2h(ctx, F_out, dout, bytes)) != BPMAX_OK) return rc;
     return bpx_collect_timing(ctx);
+}
+
+// memcpy on up to nth threads (large host copies out of pinned staging)
+static void par_memcpy(void *dst, const void *src, size_t n, unsigned nth)
+{
+    if (n < (size_t(8) << 20) || nth <= 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n + nth - 1) / nth;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nth && t * part < n; ++t) {
+        const size_t a = t * part, b = std::min(n, a + part);
+        th.emplace_back([=] { memcpy((char *)dst + a, (const char *)src + a, b - a); });
+    }
+    memcpy(dst, src, std::min(n, part));
+    for (auto &x : th) x.join();
+}
+
+// Device-to-host copy into pageable memory: in chunks through two pinned buffers, the DMA of
+// chunk c+1 running while host threads copy chunk c out (a copy straight into pageable
+// memory is staged by the runtime at a fraction of the link rate).  Synchronous.
+int bpx_d2h(bpmax_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kChunk = size_t(64) << 20;
+    if (bytes <= kChunk) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return BPMAX_OK;
+    }
+    for (int b = 0; b < 2; ++b)
+        if (!ctx->pin[b]) {
+            if (hipHostMalloc(&ctx->pin[b], kChunk) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->pin[b] = nullptr;
+                ctx->err = "hipHostMalloc of the copy staging failed";
+                return BPMAX_ENOMEM;
+            }
+            HIPCHK(ctx, hipEventCreateWithFlags(&ctx->pin_ev[b], hipEventDisableTiming));
+        }
+    const size_t n = (bytes + kChunk - 1) / kChunk;
+    auto len = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
+    auto issue = [&](size_t c) -> hipError_t {
+        const hipError_t e = hipMemcpyAsync(ctx->pin[c % 2], (const char *)src + c * kChunk, len(c),
+                                            hipMemcpyDeviceToHost, ctx->stream);
+        return e != hipSuccess ? e : hipEventRecord(ctx->pin_ev[c % 2], ctx->stream);
+    };
+    const unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    HIPCHK(ctx, issue(0));
+    for (size_t c = 0; c < n; ++c) {
+        if (c + 1 < n) HIPCHK(ctx, issue(c + 1)); // the buffer it fills was emptied at c-1
+        HIPCHK(ctx, hipEventSynchronize(ctx->pin_ev[c % 2]));
+        par_memcpy((char *)dst + c * kChunk, ctx->pin[c % 2], len(c), nth);
+    }
+    return BPMAX_OK;
 }
 
 extern "C" int bpmax_engine_solve_batch(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const int *Ms,

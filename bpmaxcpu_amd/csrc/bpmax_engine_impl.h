@@ -132,6 +132,8 @@ struct bpmax_ctx {
     std::vector<bpx::PendingTiming> pending;
     bpx::KStat stats[BPMAX_K_NCLASSES];
     std::vector<void *> user_allocs;
+    void *pin[2] = {nullptr, nullptr};   // pinned staging of large device-to-host copies
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bpx::Server *server = nullptr;
 };
 
@@ -163,4 +165,6 @@ float *bpx_arena(bpmax_ctx *ctx);
 int bpx_copy_out(bpmax_ctx *ctx, const float *arena, const bpx::PairDesc &P, bpmax_layout layout, float *F_out);
 size_t bpx_arena_bytes(int npairs, const int *Ms, const int *Ns);
 int bpx_collect_timing(bpmax_ctx *ctx);
+// Synchronous device-to-host copy of `bytes` into pageable memory, on ctx's stream.
+int bpx_d2h(bpmax_ctx *ctx, void *dst, const void *src, size_t bytes);
 void bpx_server_shutdown(bpmax_ctx *ctx);

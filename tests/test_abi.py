@@ -166,3 +166,25 @@ def test_schedule_check_catches_a_missing_wait(ahead):
             assert msg
             caught += 1
     assert caught >= 20
+
+
+def test_bench_tree6_addresses_every_level():
+    """The reference's 6-level F pointer tree that the bench builds with numpy address
+    arithmetic (src/bpmax-wrapper.c:175-197) points every leaf at the reference's offset."""
+    import ctypes
+    import bench
+    M, N, Nt = 3, 20, 8
+    lin, top, keep = bench.tree6(M, N, Nt)
+    Ns, sec = (N + Nt - 1) // Nt, (Nt + 1) * Nt
+    U = ctypes.POINTER(ctypes.c_uint64)
+    l1 = ctypes.cast(top.ctypes.data, U)
+    for a in range(M):
+        l2 = ctypes.cast(int(l1[a]), U)
+        for b in range(M):
+            l3 = ctypes.cast(int(l2[b]), U)
+            for c in range(Ns):
+                l4 = ctypes.cast(int(l3[c]), U)
+                for d in range(Ns):
+                    l5 = ctypes.cast(int(l4[d]), U)
+                    for e in range(Nt + 1):
+                        assert int(l5[e]) == lin.ctypes.data + 4 * ((((a * M + b) * Ns + c) * Ns + d) * sec + e * Nt)
