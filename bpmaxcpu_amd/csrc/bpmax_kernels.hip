@@ -48,16 +48,193 @@ struct Inner {
 //          (Bs[-1]) the unshifted tile is Bs[r-1]  -> R3  S1(i1,k1) + F(k1+1,j1)
 // so a split of the steps over work items splits R3/R4 with them.  With WI_EPILOGUE also
 // S1(i1,j1)+S2 (the inter base case on the diagonal when d1 = 0) and P1.
+// T = 64 stages operands directly into two LDS stages (DlStage); T = 32 through registers
+// (As, B's extra row, Bs).
 template <int T>
 struct OuterLds {
-    static constexpr int FLOATS = Tc<T>::PROD_FLOATS + Tc<T>::LD; // As, B's extra row, Bs
+    static constexpr int FLOATS = T == 64 ? 2 * DlStage::FLOATS : Tc<T>::PROD_FLOATS + Tc<T>::LD;
 };
+
+// Epilogue of an outer item (WI_EPILOGUE): S1(i1,j1) + S2, the inter base case on the
+// diagonal when d1 = 0, and P1 = F(i1+1,j1-1) + intra weight when d1 >= 4.
+template <int T>
+__device__ __forceinline__ void outer_epilogue(const PairDesc &P, int i1, int j1, long long toff, const float *arena,
+                                               const int *__restrict__ codes, const float *__restrict__ w,
+                                               float (&acc)[Tc<T>::R][Tc<T>::R], int r0, int c0, int I, int J)
+{
+    constexpr int R = Tc<T>::R;
+    const float *S2t = arena + P.S2 + toff;
+    const int *a = codes + P.a, *b = codes + P.b;
+    const float s1 = arena[P.S1 + i1 * P.M + j1];
+    const int d1 = j1 - i1;
+    const float w1 = d1 >= 4 ? w[5 * a[i1] + a[j1]] : 0.f;
+    const float *Pt = d1 >= 4 ? tri_at((float *)arena, P, i1 + 1, j1 - 1) + toff : S2t;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int lr = r0 + i, lc = c0 + j;
+            float v = s1 + S2t[lr * T + lc];
+            if (d1 == 0 && I * T + lr == J * T + lc) v = w[25 + 5 * a[i1] + b[I * T + lr]]; // inter base
+            if (d1 >= 4) v = fmaxf(v, Pt[lr * T + lc] + w1);
+            acc[i][j] = fmaxf(acc[i][j], v);
+        }
+}
+
+// The item's own and helper k ranges (see outer_body_rs): own [mb, me) and, for a wave that
+// helps, wave hw's cells over [hb, he); helped: this wave receives a helper's slice.
+struct OuterRanges {
+    int mb = 0, me = 64, hw = -1, hb = 0, he = 0;
+    bool helped = false, balance = false;
+};
+template <int T, bool BAL>
+__device__ __forceinline__ OuterRanges outer_ranges(const WorkItem &it, int nK, int Kb, int wv)
+{
+    constexpr int QS = Tc<T>::QS, H = QS / 2;
+    const int qr = wv >> 1, qc = wv & 1, I = it.I, J = it.J;
+    OuterRanges g;
+    g.me = T;
+    g.balance = BAL && nK == 1 && (Kb == I || Kb == J) && it.step_begin < it.step_end;
+    if (!g.balance) return g;
+    if (Kb == I && Kb == J) { // w0 [0,QS)  w1 [0,2QS) -> w1 [QS,2QS) + w2 [0,QS)  w3 [QS,2QS)
+        g.mb = wv == 3 || wv == 1 ? QS : 0;
+        g.me = wv == 0 ? QS : (wv == 2 ? 0 : T);
+        if (wv == 2) g.hw = 1, g.hb = 0, g.he = QS;
+        g.helped = wv == 1;
+    } else if (Kb == I) { // rows of qr = 1 need k >= QS: qr 0 [H,2QS), qr 1 [QS,2QS) + [0,H) of (0,qc)
+        g.mb = qr ? QS : H;
+        if (qr) g.hw = qc, g.hb = 0, g.he = H;
+        g.helped = qr == 0;
+    } else { // columns of qc = 0 need k < QS: qc 1 [0,2QS-H), qc 0 [0,QS) + [2QS-H,2QS) of (qr,1)
+        g.me = qc ? T - H : QS;
+        if (!qc) g.hw = 2 * qr + 1, g.hb = T - H, g.he = T;
+        g.helped = qc == 1;
+    }
+    return g;
+}
+
+// End of a balanced item: helpers hand their slices (acc2) to the helped waves through LDS.
+template <int T>
+__device__ __forceinline__ void outer_merge(const OuterRanges &g, float (&acc)[Tc<T>::R][Tc<T>::R],
+                                            const float (&acc2)[Tc<T>::R][Tc<T>::R], float *smem, int wv)
+{
+    constexpr int R = Tc<T>::R, RG = R * R * 64;
+    const int l = threadIdx.x % 64;
+    __syncthreads();
+    if (g.hw >= 0) {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j) smem[g.hw * RG + i * R * 64 + l * R + j] = acc2[i][j];
+    }
+    __syncthreads();
+    if (g.helped) {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], smem[wv * RG + i * R * 64 + l * R + j]);
+    }
+}
+
+// 64 x 64 storage tiles: operands staged straight into LDS (DlStage), two 32-deep halves per
+// K step.  Half h = 2s + kh of step s: A = F(i1,k1) tile (I,K) columns 32kh.., B = rows
+// K*64 + 32kh + q (q = 0..32) of F(k1+1,j1), tile column J (row 64 is row 0 of tile K+1, or
+// -inf when K = J); R0 reads B row k+1 for k (the k2+1 shift).  R3 (K = I) reads the
+// unshifted B rows of its half, R4 (K = J) the A columns of its half; their S1 terms are
+// staged with the first half of the step.
+template <bool BAL>
+__device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
+                              const int *__restrict__ codes, const float *__restrict__ w, float *smem)
+{
+    constexpr int T = 64, R = 4, QS = Tc<T>::QS, H = QS / 2;
+    int r0, c0;
+    lane_block<T>(r0, c0);
+    const int i1 = it.i1, j1 = it.j1, I = it.I, J = it.J, nb = P.nb, M = P.M;
+    const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
+    const float *S1 = arena + P.S1;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
+    const int kcnt = (it.flags >> WI_KCNT_SHIFT) & 255;
+    const int nK = kcnt ? kcnt : J - I + 1, Kb = I + ((it.flags >> WI_KOFF_SHIFT) & 255);
+    float acc[R][R], acc2[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[i][j] = acc2[i][j] = NEG_INF;
+    const OuterRanges g = outer_ranges<T, BAL>(it, nK, Kb, wv);
+    int hr0 = 0, hc0 = 0;
+    if (g.hw >= 0) lane_block<T>(hr0, hc0, g.hw * 64 + l);
+    // padding (N not a multiple of T): see outer_body_rs
+    const int kreal = (P.N - (nb - 1) * T + H - 1) / H * H;
+    auto quad_pad = [&](int x) {
+        return (I == nb - 1 && I * T + (x >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (x & 1) * QS >= P.N);
+    };
+    const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
+    auto issue = [&](int h) {
+        const int s = h >> 1, kh = h & 1, k1 = i1 + s / nK, K = Kb + s % nK;
+        float *st = smem + (h & 1) * DlStage::FLOATS;
+        const float *Bf = tri_at(arena, P, k1 + 1, j1);
+        dl_stage_A(st, tri_at(arena, P, i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), kh, wv, l);
+        dl_stage_B32(st, Bf + utri_ll(K, J, nb) * (long long)(T * T), kh, wv, l);
+        if (wv == 0 && l < 16) { // B row 32 of the half: row 32 of tile K, or row 0 of tile K+1 (-inf past J)
+            const float *src = kh == 0 ? Bf + utri_ll(K, J, nb) * (long long)(T * T) + 32 * T
+                                       : (K + 1 <= J ? Bf + utri_ll(K + 1, J, nb) * (long long)(T * T)
+                                                     : (const float *)kNegInfRow);
+            g2lds16(src + 4 * l, st + DlStage::B + 32 * 64);
+        }
+        if (kh == 0 && l == 0) {
+            if (wv == 1 && K == I) g2lds4(S1 + i1 * M + k1, st + DlStage::S1);
+            if (wv == 2 && K == J) g2lds4(S1 + (k1 + 1) * M + j1, st + DlStage::S1 + 1);
+        }
+    };
+    const int h0 = 2 * it.step_begin, h1 = 2 * it.step_end;
+    if (h0 < h1) issue(h0);
+    float s1a = NEG_INF, s1b = NEG_INF;
+    for (int h = h0; h < h1; ++h) {
+        const int s = h >> 1, kh = h & 1, K = Kb + s % nK;
+        const float *st = smem + (h & 1) * DlStage::FLOATS;
+        vm_wait_all();
+        __syncthreads();
+        if (kh == 0) { // read before the next loads are issued into the other stage
+            s1a = st[DlStage::S1];
+            s1b = st[DlStage::S1 + 1];
+        }
+        if (h + 1 < h1) issue(h + 1); // in flight while this half computes
+        if (K == J && kh == (wv & 1)) { // R4: A(r, c) = F(i1,k1)[I*T + r][J*T + c]
+            const float4 *A = (const float4 *)(st + DlStage::A) + r0 * 8 + ((l % 8) ^ ((r0 >> 2) & 7));
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const float4 v = A[i * 8];
+                const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], e[j] + s1b);
+            }
+        }
+        if (K == I && kh == (wv >> 1)) { // R3: B row (r - 32kh) = F(k1+1,j1)[I*T + r][J*T + c]
+            const float *B = st + DlStage::B + (r0 - 32 * kh) * 64 + c0;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const float4 v = *(const float4 *)(B + i * 64);
+                const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], s1a + e[j]);
+            }
+        }
+        int kb = g.mb, ke = g.me;
+        if (!g.balance) quad_k_range<T>(K == I, K == J, false, kb, ke);
+        const int klim = K == nb - 1 ? kreal : T;
+        if (!own_pad) dl_product<1>(acc, st, r0, c0, kh, kb, ke < klim ? ke : klim);
+        if (g.hw >= 0 && !help_pad) dl_product<1>(acc2, st, hr0, hc0, kh, g.hb, g.he < klim ? g.he : klim);
+    }
+    if (g.balance) outer_merge<T>(g, acc, acc2, smem, wv);
+    if (it.flags & WI_EPILOGUE) outer_epilogue<T>(P, i1, j1, toff, arena, codes, w, acc, r0, c0, I, J);
+    store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, r0, c0, (it.flags & WI_ATOMIC) != 0);
+}
 
 // BAL: balance the waves' k ranges on diagonal steps (below); off in the task kernel, whose
 // register budget it would exceed.
-template <int T, bool BAL = true>
-__device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
-                           const int *__restrict__ codes, const float *__restrict__ w, float *smem)
+template <int T, bool BAL>
+__device__ void outer_body_rs(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
+                              const int *__restrict__ codes, const float *__restrict__ w, float *smem)
 {
     constexpr int LD = Tc<T>::LD, R = Tc<T>::R, Q = Stage<T>::Q;
     float *As = smem, *Bs = smem + (T + 1) * LD;
@@ -201,6 +378,14 @@ __device__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restr
             }
     }
     store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, r0, c0, (it.flags & WI_ATOMIC) != 0);
+}
+
+template <int T, bool BAL = true>
+__device__ __forceinline__ void outer_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
+                                           const int *__restrict__ codes, const float *__restrict__ w, float *smem)
+{
+    if constexpr (T == 64) outer_body_dl<BAL>(P, it, arena, codes, w, smem);
+    else outer_body_rs<T, BAL>(P, it, arena, codes, w, smem);
 }
 
 // ------------------------------------------------------------------------ inner ext
@@ -1057,11 +1242,72 @@ __global__ void k_scores(const PairDesc *__restrict__ pairs, int npairs, const f
 // ------------------------------------------------------------------- isolated R0
 // C[i][j] max= max_{i<=k<=j} A[i][k] + B[k][j] on a batch of tiled upper triangles;
 // block = (output tile index within the triangle, matrix).  tiles holds the (I,J) list.
+// 64 x 64 tiles: operands staged straight into two LDS stages (DlStage), two 32-deep halves
+// per K step, B unshifted.
+__device__ __forceinline__ void r0tri_dl(int nb, const int2 *__restrict__ tiles, const float *__restrict__ A,
+                                         const float *__restrict__ B, float *__restrict__ C)
+{
+    constexpr int T = 64, R = 4;
+    __shared__ __attribute__((aligned(16))) float smem[2 * DlStage::FLOATS];
+    const int wi = xcd_work_index(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int2 IJ = tiles[wi % gridDim.x];
+    const int I = IJ.x, J = IJ.y;
+    const long long tstride = tri_ll(nb) * T * T;
+    const long long m = wi / gridDim.x;
+    const float *Am = A + m * tstride, *Bm = B + m * tstride;
+    float *Cm = C + m * tstride;
+    int r0, c0;
+    lane_block<T>(r0, c0);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64, hw = balance_target(wv);
+    float acc[R][R], acc2[R][R];
+    const float *Ct = tile_at(Cm, I, J, nb, T);
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            acc[i][j] = Ct[(r0 + i) * T + c0 + j];
+            acc2[i][j] = NEG_INF;
+        }
+    int hr0 = 0, hc0 = 0;
+    if (hw >= 0) lane_block<T>(hr0, hc0, hw * 64 + l);
+    auto issue = [&](int h) {
+        const int K = I + (h >> 1), kh = h & 1;
+        float *st = smem + (h & 1) * DlStage::FLOATS;
+        dl_stage_A(st, tile_at(Am, I, K, nb, T), kh, wv, l);
+        dl_stage_B32(st, tile_at(Bm, K, J, nb, T), kh, wv, l);
+    };
+    const int nh = 2 * (J - I + 1);
+    issue(0);
+    // Wait for C here: the compiler counts only its own loads, so a wait it placed at acc's
+    // first use inside the loop would also wait for the stage loads in flight there.
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) asm volatile("" : "+v"(acc[i][j]));
+    for (int h = 0; h < nh; ++h) {
+        const int K = I + (h >> 1), kh = h & 1;
+        vm_wait_all();
+        __syncthreads();
+        if (h + 1 < nh) issue(h + 1);
+        const float *st = smem + (h & 1) * DlStage::FLOATS;
+        int kb, ke, hb, he;
+        balanced_k_range<T>(K == I, K == J, wv, kb, ke, hb, he);
+        dl_product<0>(acc, st, r0, c0, kh, kb, ke);
+        if (hw >= 0) dl_product<0>(acc2, st, hr0, hc0, kh, hb, he);
+    }
+    balance_merge<T>(acc, acc2, smem, wv);
+    store_acc<T, T>(tile_at(Cm, I, J, nb, T), acc, r0, c0, false);
+}
+
 template <int T>
 __global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restrict__ tiles,
                                                     const float *__restrict__ A, const float *__restrict__ B,
                                                     float *__restrict__ C)
 {
+    if constexpr (T == 64) {
+        r0tri_dl(nb, tiles, A, B, C);
+        return;
+    }
     constexpr int LD = Tc<T>::LD, R = Tc<T>::R;
     __shared__ __attribute__((aligned(16))) float As[T * LD];
     __shared__ __attribute__((aligned(16))) float Bs[T * LD];

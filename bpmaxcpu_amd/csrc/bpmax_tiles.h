@@ -357,4 +357,115 @@ __device__ __forceinline__ void max3_into(float &acc, float x, float y)
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// ------------------------------------------------------- direct-to-LDS staging (64 x 64 R0)
+// The 64 x 64 R0 product steps stage their operands with global_load_lds_dwordx4: one
+// wave-wide load writes 16 bytes per lane straight into LDS at a wave-uniform base + 16 *
+// lane, so tiles reach LDS without passing through VGPRs or a commit pass.  A 64-deep K
+// step runs as two 32-deep halves on two LDS stages: the loads of half h+1 fly while half h
+// computes, one barrier per half.
+//
+// Stage layout (floats):
+//   A   64 rows x 8 float4: A[r][32*kh + 4*k4 .. +3] at float4 slot r*8 + (k4 ^ ((r >> 2) & 7))
+//       (row-major 32-wide halves; k4 swizzled per 4-row group, so that the eight 4-row
+//       groups a wave reads at one k4 fall in eight different bank groups)
+//   B   33 rows x 64: rows 32*kh .. 32*kh+32 of the unshifted B operand, row-major (the
+//       33rd row serves the k2+1 shift of R0's last k of the half)
+//   S1  4 floats: scalar terms that travel with the stage
+struct DlStage {
+    static constexpr int A = 0, B = 64 * 32, S1 = B + 33 * 64, FLOATS = S1 + 4;
+};
+static_assert(DlStage::FLOATS % 4 == 0, "stages stay 16-byte aligned");
+
+// A row of -inf in device memory: the source of staged rows that lie outside the triangle.
+static __device__ __attribute__((aligned(256))) const unsigned kNegInfRow[64] = {
+#define BPX_NI4 0xFF800000u, 0xFF800000u, 0xFF800000u, 0xFF800000u
+#define BPX_NI16 BPX_NI4, BPX_NI4, BPX_NI4, BPX_NI4
+    BPX_NI16, BPX_NI16, BPX_NI16, BPX_NI16
+#undef BPX_NI16
+#undef BPX_NI4
+};
+
+__device__ __forceinline__ unsigned lds_offset(const void *p)
+{
+    // the low 32 bits of a generic address in the LDS aperture are the LDS byte offset
+    return __builtin_amdgcn_readfirstlane((unsigned)(size_t)p);
+}
+
+// 16 (g2lds16) or 4 (g2lds4) bytes per active lane from global address g (per lane) into LDS
+// at lds + lane * size (lds wave-uniform).  Written as asm: with the compiler's builtin it
+// orders every later LDS read of the wave behind these loads (it cannot tell the two stages
+// apart), which serialises the prefetch with the product.  These loads have no register
+// result; a stage is published by vm_wait_all() and a barrier.
+__device__ __forceinline__ void g2lds16(const void *g, const void *lds)
+{
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(lds_offset(lds)) : "memory", "m0");
+}
+__device__ __forceinline__ void g2lds4(const void *g, const void *lds)
+{
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" : : "v"(g), "s"(lds_offset(lds)) : "memory", "m0");
+}
+__device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" : : : "memory"); }
+
+// Stage the A half [32*kh, 32*kh+32) of the 64 x 64 tile At (row stride 64): wave w loads
+// rows 16w .. 16w+15, lane l row 16w + 8c + l/8 at swizzled slot l%8.
+__device__ __forceinline__ void dl_stage_A(float *stage, const float *At, int kh, int w, int l)
+{
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int r = 16 * w + 8 * c + l / 8, k4 = (l % 8) ^ ((r >> 2) & 7);
+        g2lds16(At + r * 64 + 32 * kh + 4 * k4, stage + DlStage::A + (2 * w + c) * 256);
+    }
+}
+
+// Stage rows 32*kh + q, q in [8w, 8w+8), of the 64 x 64 tile Bt: lane l row 8w + 4c + l/16,
+// columns 4*(l%16) ..
+__device__ __forceinline__ void dl_stage_B32(float *stage, const float *Bt, int kh, int w, int l)
+{
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int q = 8 * w + 4 * c + l / 16;
+        g2lds16(Bt + (32 * kh + q) * 64 + 4 * (l % 16), stage + DlStage::B + (8 * w + 4 * c) * 64);
+    }
+}
+
+// acc[i][j] max= A[r0+i][k] + B[k+RS][c0+j] (stage rows) over the 16 k of the stage's half
+// from kq (0 or 16): per 4 k, four A float4 (the lane's rows) and four B float4 (its
+// columns) for 64 updates, v_add_f32 x 2 + v_max3_f32 per pair.
+template <int RS>
+__device__ __forceinline__ void dl_prod16(float (&acc)[4][4], const float *stage, int r0, int c0, int kq)
+{
+    const float4 *A = (const float4 *)(stage + DlStage::A) + r0 * 8;
+    const float *B = stage + DlStage::B + (kq + RS) * 64 + c0;
+    const int sw = ((r0 >> 2) & 7) ^ (kq >> 2);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        float4 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = A[i * 8 + (sw ^ u)];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) b[v] = *(const float4 *)(B + (4 * u + v) * 64);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float av[4] = {a[i].x, a[i].y, a[i].z, a[i].w};
+            const float b0[4] = {b[0].x, b[0].y, b[0].z, b[0].w}, b1[4] = {b[1].x, b[1].y, b[1].z, b[1].w};
+            const float b2[4] = {b[2].x, b[2].y, b[2].z, b[2].w}, b3[4] = {b[3].x, b[3].y, b[3].z, b[3].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = max3(acc[i][j], av[0] + b0[j], av[1] + b1[j]);
+                acc[i][j] = max3(acc[i][j], av[2] + b2[j], av[3] + b3[j]);
+            }
+        }
+    }
+}
+
+// dl_prod16 over [lo, hi) (multiples of 16) clipped to the stage's half [32*kh, 32*kh+32).
+template <int RS>
+__device__ __forceinline__ void dl_product(float (&acc)[4][4], const float *stage, int r0, int c0, int kh, int lo,
+                                           int hi)
+{
+    lo = lo > 32 * kh ? lo : 32 * kh;
+    hi = hi < 32 * kh + 32 ? hi : 32 * kh + 32;
+    for (int k = lo; k < hi; k += 16) dl_prod16<RS>(acc, stage, r0, c0, k - 32 * kh);
+}
+
 } // namespace bpx
