@@ -153,6 +153,7 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
     }
     if (const char *e = getenv("BPMAX_OUTER_SPLIT")) c->outer_split = std::max(0, atoi(e));
     if (const char *e = getenv("BPMAX_OUTER_KGROUP")) c->outer_kgroup = std::max(0, atoi(e));
+    if (const char *e = getenv("BPMAX_R0_AHEAD")) c->r0_ahead = atoi(e) != 0;
     if (const char *e = getenv("BPMAX_CLOSURE_AHEAD")) c->closure_ahead = atoi(e) != 0;
     if (const char *e = getenv("BPMAX_INTRA_BLOCKED")) c->intra_blocked = atoi(e) != 0;
     *out = c;
@@ -639,14 +640,22 @@ void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g
     }
 }
 
-void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nbmax, int nblmax,
-               int tick, Schedule &S, std::vector<int> &out_end, int outer_split, int outer_items,
-               int outer_kgroup, bool closure_ahead)
+// Which k1 terms of a triangle on outer diagonal d1 an R0 launch covers.
+enum R0Part {
+    R0_ALL,  // every k1 in [i1, j1), with the epilogue
+    R0_BULK, // k1 in [i1+1, j1-2]: operands on outer diagonals <= d1-2
+    R0_EDGE, // k1 = i1 and k1 = j1-1 (operands on diagonal d1-1), with the epilogue
+};
+
+// R0 launches of one step of the outer wavefront (see emit_tick) covering the k1 terms of
+// `part` (R0_EDGE: both terms of a tile in one item of two steps per K), in `outer_split`
+// ranges of storage diagonals, whose ends out_end receives.
+static void emit_r0(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nbmax, int tick,
+                    Schedule &S, std::vector<int> &out_end, int outer_split, int outer_items, int outer_kgroup,
+                    R0Part part)
 {
     auto active = [&](size_t p) { return P[p].T == T && d1s[p] >= 0 && d1s[p] < P[p].M; };
-    auto emit = [&](int kind, size_t first, double upd, int diag) {
-        S.launches.push_back(Launch{kind, T, false, (int)first, (int)(S.items.size() - first), upd, g, tick, diag});
-    };
+    auto nk1 = [&](int d1) { return part == R0_ALL ? d1 : (part == R0_BULK ? std::max(0, d1 - 2) : std::min(d1, 2)); };
     // R0 launches: storage diagonals in `split` contiguous ranges
     const int split = outer_split <= 0 ? nbmax : std::min(outer_split, nbmax);
     std::vector<int> range_end(split);
@@ -672,11 +681,11 @@ void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int 
             long long tiles = 0;
             for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
                 tiles += (long long)(P[p].M - d1) * (P[p].nb - Ds) * groups(kg, Ds + 1);
-            steps += (long long)d1 * tiles;
+            steps += (long long)nk1(d1) * tiles;
             if (r == 0) {
                 const double n = P[p].N;
                 const double c3n = (n + 1) * n * (n - 1) / 6.0, c2n = (n + 1) * n / 2.0;
-                upd += (double)(P[p].M - d1) * d1 * (c3n + 2.0 * c2n);
+                upd += (double)(P[p].M - d1) * nk1(d1) * (c3n + 2.0 * c2n);
             }
         }
         // Items cover whole k1 values (ck of them, every K of each), ordered triangle,
@@ -686,8 +695,10 @@ void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int 
         const int ck = (int)std::max<long long>(1, (steps + want - 1) / want);
         for (size_t p = 0; p < P.size(); ++p) {
             if (!active(p)) continue;
-            const int d1 = d1s[p], kg = kg_of(d1);
-            const int nck = d1 == 0 ? 1 : (d1 + ck - 1) / ck;
+            const int d1 = d1s[p], kg = kg_of(d1), n1 = nk1(d1);
+            const int k1off = part == R0_BULK ? 1 : 0; // first k1 of the part, minus i1
+            // k1 chunks; R0_ALL on d1 = 0 and R0_EDGE: one (the epilogue, or both edge terms)
+            const int nck = part == R0_EDGE ? 1 : (n1 == 0 ? (part == R0_ALL ? 1 : 0) : (n1 + ck - 1) / ck);
             for (int i1 = 0; i1 + d1 < P[p].M; ++i1)
                 for (int c = 0; c < nck; ++c)
                     for (int Ds = Da; Ds <= Db && Ds < P[p].nb; ++Ds)
@@ -695,15 +706,38 @@ void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int 
                             for (int gi = 0, ng = groups(kg, Ds + 1); gi < ng; ++gi) {
                                 const int koff = kg > 0 ? gi * kg : 0;
                                 const int nK = kg > 0 ? std::min(kg, Ds + 1 - koff) : Ds + 1;
-                                const int sb = c * ck * nK, se = std::min(d1, (c + 1) * ck) * nK;
-                                int fl = (nck > 1 || ng > 1 ? WI_ATOMIC : 0) | (c == 0 && gi == 0 ? WI_EPILOGUE : 0);
+                                int sb, se, fl;
+                                if (part == R0_EDGE) {
+                                    // d1 = 0: the epilogue alone; d1 = 1: k1 = i1; else both edges
+                                    sb = 0;
+                                    se = std::min(d1, 2) * nK;
+                                    fl = (d1 >= 3 || ng > 1 ? WI_ATOMIC : 0) | (gi == 0 ? WI_EPILOGUE : 0) |
+                                         (d1 >= 2 ? WI_EDGE : 0);
+                                } else {
+                                    sb = (k1off + c * ck) * nK;
+                                    se = (k1off + std::min(n1, (c + 1) * ck)) * nK;
+                                    fl = part == R0_BULK ? WI_ATOMIC
+                                                         : (nck > 1 || ng > 1 ? WI_ATOMIC : 0) |
+                                                               (c == 0 && gi == 0 ? WI_EPILOGUE : 0);
+                                }
                                 if (kg > 0) fl |= (koff << WI_KOFF_SHIFT) | (nK << WI_KCNT_SHIFT);
                                 S.items.push_back(WorkItem{(int)p, i1, i1 + d1, I, I + Ds, sb, se, fl});
                             }
         }
         split_tail(S.items, f);
-        emit(BPMAX_K_OUTER, f, upd, Db);
+        S.launches.push_back(Launch{BPMAX_K_OUTER, T, false, (int)f, (int)(S.items.size() - f), upd, g, tick, Db});
+        S.launches.back().bulk = part == R0_BULK;
     }
+}
+
+// Inner-closure launches of one step of the outer wavefront (see emit_tick).
+static void emit_closure(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nblmax,
+                         int tick, Schedule &S, bool closure_ahead)
+{
+    auto active = [&](size_t p) { return P[p].T == T && d1s[p] >= 0 && d1s[p] < P[p].M; };
+    auto emit = [&](int kind, size_t first, double upd, int diag) {
+        S.launches.push_back(Launch{kind, T, false, (int)first, (int)(S.items.size() - first), upd, g, tick, diag});
+    };
     // inner closure by logical block diagonal
     for (int D = 0; D < nblmax; ++D) {
         if (D > 0 && !closure_ahead) {
@@ -754,11 +788,20 @@ void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int 
     }
 }
 
+
+void emit_tick(const std::vector<PairDesc> &P, const std::vector<int> &d1s, int g, int T, int nbmax, int nblmax,
+               int tick, Schedule &S, std::vector<int> &out_end, int outer_split, int outer_items,
+               int outer_kgroup, bool closure_ahead)
+{
+    emit_r0(P, d1s, g, T, nbmax, tick, S, out_end, outer_split, outer_items, outer_kgroup, R0_ALL);
+    emit_closure(P, d1s, g, T, nblmax, tick, S, closure_ahead);
+}
+
 // The launch list of a whole solve of a set of pairs (descs already laid out), one group per
 // storage tile size T: S2 of every pair, then one tick per outer diagonal d1 with every
 // pair on d1 (pairs with M <= d1 are done).
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
-                    bool closure_ahead)
+                    bool closure_ahead, bool r0_ahead)
 {
     S.items.clear();
     S.launches.clear();
@@ -781,10 +824,29 @@ void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split
     for (int g = 0; g < (int)S.groups.size(); ++g) {
         GroupInfo &G = S.groups[g];
         emit_s2(P, all, g, G.T, G.maxnbl, S);
+        if (!r0_ahead) {
+            for (int d1 = 0; d1 < G.maxM; ++d1) {
+                std::fill(d1s.begin(), d1s.end(), d1);
+                emit_tick(P, d1s, g, G.T, G.maxnb, G.maxnbl, d1, S, G.out_end, outer_split, outer_items,
+                          outer_kgroup, closure_ahead);
+            }
+            continue;
+        }
+        // Step d1: the edge R0 of d1 (it reads diagonal d1-1), the bulk R0 of d1+1 (it reads
+        // diagonals <= d1-1), then the closure of d1.  Both R0 parts become ready when the
+        // closure of d1-1 ends; the edge part, which the closure of d1 waits for, goes first
+        // on the R0 stream, and the bulk of d1+1 then runs beside the closure of d1.  Every
+        // launch a later one waits for is emitted before it.
         for (int d1 = 0; d1 < G.maxM; ++d1) {
             std::fill(d1s.begin(), d1s.end(), d1);
-            emit_tick(P, d1s, g, G.T, G.maxnb, G.maxnbl, d1, S, G.out_end, outer_split, outer_items, outer_kgroup,
-                      closure_ahead);
+            emit_r0(P, d1s, g, G.T, G.maxnb, d1, S, G.out_end, outer_split, outer_items, outer_kgroup, R0_EDGE);
+            if (d1 + 1 < G.maxM) {
+                std::fill(d1s.begin(), d1s.end(), d1 + 1);
+                emit_r0(P, d1s, g, G.T, G.maxnb, d1 + 1, S, G.out_end, outer_split, outer_items, outer_kgroup,
+                        R0_BULK);
+            }
+            std::fill(d1s.begin(), d1s.end(), d1);
+            emit_closure(P, d1s, g, G.T, G.maxnbl, d1, S, closure_ahead);
         }
     }
 }
@@ -977,7 +1039,7 @@ int bpx_run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDe
 static int ensure_sync_events(bpmax_ctx *ctx, const Schedule &S, std::vector<size_t> &out_base,
                               std::vector<size_t> &in_base)
 {
-    size_t n = 3;
+    size_t n = 3; // fill, S2, end
     out_base.assign(S.groups.size(), 0);
     in_base.assign(S.groups.size(), 0);
     for (size_t g = 0; g < S.groups.size(); ++g) {
@@ -1003,14 +1065,19 @@ static int closure_r0_diag(const GroupInfo &G, const Launch &L)
     return std::min((L.diag + (L.ahead ? 1 : 0) + G.rs - 1) / G.rs, G.maxnb - 1);
 }
 
-// Pipelined schedule on two streams: sO runs the fill and every R0 launch, sI runs S1, S2
+// Pipelined schedule on two streams: sO runs the fill, S1 and the R0 launches, sI runs S2
 // and every inner-closure launch.  R0 of (d1, storage diagonal Ds) reads tiles of the
 // triangles on diagonals < d1 up to storage diagonal Ds, i.e. logical diagonal
-// rs*Ds + rs-1 -- it waits for the closure of d1-1 to reach that logical diagonal (for
-// d1 = 0: for S1/S2).  The closure of (d1, logical diagonal DL) needs R0 done on every
-// storage tile that holds a logical tile of diagonal DL, i.e. storage diagonal
+// rs*Ds + rs-1 -- a whole R0 launch waits for the closure of d1-1 to reach that logical
+// diagonal (for d1 = 0: for S2).  The closure of (d1, logical diagonal DL) needs R0 done on
+// every storage tile that holds a logical tile of diagonal DL, i.e. storage diagonal
 // ceil(DL/rs).  The closure of d1 thus overlaps the R0 launches of d1 (later storage
 // diagonals) and of d1+1 (earlier ones).
+// With the R0 of d1 split (build_schedule's r0_ahead), its bulk launches (k1 terms whose
+// operands lie on diagonals <= d1-2) wait for the closure of d1-2 instead, and run beside
+// the closure of d1-1; its edge launches (k1 = i1 and k1 = j1-1) wait for the closure of
+// d1-1 like whole R0 launches.  The closure of d1 waits for the edge launches, which follow
+// the bulk ones of d1 on sO.
 static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const PairDesc *dP, const WorkItem *dI,
                         float *arena, long long fl, const int *dcodes, const float *dw)
 {
@@ -1021,16 +1088,18 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
     hipEvent_t ev_fill = ctx->sync_ev[0], ev_s2 = ctx->sync_ev[1], ev_end = ctx->sync_ev[2];
     auto ev_out = [&](int g, int d1, int D) { return ctx->sync_ev[ob[g] + (size_t)d1 * S.groups[g].maxnb + D]; };
     auto ev_in = [&](int g, int d1, int D) { return ctx->sync_ev[ib[g] + (size_t)d1 * S.groups[g].maxnbl + D]; };
-    // logical diagonals some R0 launch waits for: only those record an event (a record
-    // between two closure launches costs a few microseconds on the critical path)
     auto in_need = [&](const Launch &L) {
         const GroupInfo &G = S.groups[L.group];
         return std::min(G.rs * L.diag + G.rs - 1, G.maxnbl - 1);
     };
+    auto lag = [](const Launch &L) { return L.bulk ? 2 : 1; };
+    auto on_r0_stream = [](const Launch &L) { return L.kind == BPMAX_K_OUTER; };
+    // logical diagonals some R0 launch waits for: only those record an event (a record
+    // between two closure launches costs a few microseconds on the critical path)
     std::vector<std::vector<char>> waited(S.groups.size());
     for (size_t g = 0; g < S.groups.size(); ++g) waited[g].assign(S.groups[g].maxnbl, 0);
     for (const Launch &L : S.launches)
-        if (L.kind == BPMAX_K_OUTER && L.d1 > 0) waited[L.group][in_need(L)] = 1;
+        if (L.d1 >= 0 && on_r0_stream(L) && L.d1 - lag(L) >= 0) waited[L.group][in_need(L)] = 1;
 
     if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0, [&] { return launch_fill(sO, arena, fl, -INFINITY); })) != BPMAX_OK)
         return rc;
@@ -1044,33 +1113,32 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
         if (L.d1 < 0 && (rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
     HIPCHK(ctx, hipEventRecord(ev_s2, sI));
 
-    int cur_g = -1, cur_d1 = -1, in_wait = -1, out_wait = -1;
+    int o_g = -1, o_src = -2, o_wait = -1; // waits issued on sO for the closure of (group, o_src)
+    int i_g = -1, i_d1 = -1, i_wait = -1;  // waits issued on sI for the R0 of (group, i_d1)
     for (const Launch &L : S.launches) {
         if (L.d1 < 0) continue;
         const GroupInfo &G = S.groups[L.group];
-        if (L.group != cur_g || L.d1 != cur_d1) {
-            cur_g = L.group;
-            cur_d1 = L.d1;
-            in_wait = out_wait = -1;
-        }
-        if (L.kind == BPMAX_K_OUTER) {
-            if (L.d1 == 0) {
-                if (in_wait < 0) HIPCHK(ctx, hipStreamWaitEvent(sO, ev_s2, 0));
-                in_wait = G.maxnbl;
+        if (on_r0_stream(L)) {
+            const int src = L.d1 - lag(L);
+            if (L.group != o_g || std::max(src, -1) != o_src) o_g = L.group, o_src = std::max(src, -1), o_wait = -1;
+            if (src < 0) {
+                if (o_wait < 0) HIPCHK(ctx, hipStreamWaitEvent(sO, ev_s2, 0));
+                o_wait = G.maxnbl;
             } else {
                 const int need = in_need(L);
-                if (need > in_wait) {
-                    HIPCHK(ctx, hipStreamWaitEvent(sO, ev_in(L.group, L.d1 - 1, need), 0));
-                    in_wait = need;
+                if (need > o_wait) {
+                    HIPCHK(ctx, hipStreamWaitEvent(sO, ev_in(L.group, src, need), 0));
+                    o_wait = need;
                 }
             }
             if ((rc = run_launch(ctx, sO, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
             HIPCHK(ctx, hipEventRecord(ev_out(L.group, L.d1, L.diag), sO));
         } else {
+            if (L.group != i_g || L.d1 != i_d1) i_g = L.group, i_d1 = L.d1, i_wait = -1;
             const int need = G.out_end[closure_r0_diag(G, L)];
-            if (need > out_wait) {
+            if (need > i_wait) {
                 HIPCHK(ctx, hipStreamWaitEvent(sI, ev_out(L.group, L.d1, need), 0));
-                out_wait = need;
+                i_wait = need;
             }
             if ((rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
             if (L.kind == BPMAX_K_INNER_INTRA && waited[L.group][L.diag])
@@ -1166,8 +1234,11 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
     }
     const bool one = t->schedule == BPMAX_SCHED_LAUNCH;
     Schedule S;
-    build_schedule(P, S, one ? 1 : t->outer_split, t->outer_items, t->outer_kgroup, t->closure_ahead != 0);
+    build_schedule(P, S, one ? 1 : t->outer_split, t->outer_items, t->outer_kgroup, t->closure_ahead != 0,
+                   !one && t->schedule == BPMAX_SCHED_PIPE);
     const int n = (int)S.launches.size();
+    // R0 launches on the R0 stream (sO); everything else runs on the closure stream
+    auto on_o = [&](int i) { return !one && S.launches[i].kind == BPMAX_K_OUTER; };
     // the launch each launch waits for (-1: none), as enqueue_pipe derives it
     std::vector<int> wait(n, -1);
     if (!one) {
@@ -1176,35 +1247,33 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
         for (int i = 0; i < n; ++i) {
             const Launch &L = S.launches[i];
             if (L.d1 < 0) s2_end[L.group] = i;
-            else if (L.kind == BPMAX_K_OUTER) outer_of[{L.group, L.d1, L.diag}] = i;
+            else if (on_o(i)) outer_of[{L.group, L.d1, L.diag}] = i;
             else if (L.kind == BPMAX_K_INNER_INTRA) intra_of[{L.group, L.d1, L.diag}] = i;
         }
-        int cur_g = -1, cur_d1 = -1, in_wait = -1, out_wait = -1;
+        int o_g = -1, o_src = -2, o_wait = -1, i_g = -1, i_d1 = -1, i_wait = -1;
         for (int i = 0; i < n; ++i) {
             const Launch &L = S.launches[i];
             if (L.d1 < 0) continue;
             const GroupInfo &G = S.groups[L.group];
-            if (L.group != cur_g || L.d1 != cur_d1) {
-                cur_g = L.group;
-                cur_d1 = L.d1;
-                in_wait = out_wait = -1;
-            }
-            if (L.kind == BPMAX_K_OUTER) {
-                if (L.d1 == 0) {
-                    if (in_wait < 0) wait[i] = s2_end[L.group];
-                    in_wait = G.maxnbl;
+            if (on_o(i)) {
+                const int src = L.d1 - (L.bulk ? 2 : 1);
+                if (L.group != o_g || std::max(src, -1) != o_src) o_g = L.group, o_src = std::max(src, -1), o_wait = -1;
+                if (src < 0) {
+                    if (o_wait < 0) wait[i] = s2_end[L.group];
+                    o_wait = G.maxnbl;
                 } else {
                     const int need = std::min(G.rs * L.diag + G.rs - 1, G.maxnbl - 1);
-                    if (need > in_wait) {
-                        wait[i] = intra_of.at({L.group, L.d1 - 1, need});
-                        in_wait = need;
+                    if (need > o_wait) {
+                        wait[i] = intra_of.at({L.group, src, need});
+                        o_wait = need;
                     }
                 }
-            } else {
+            } else if (L.kind != BPMAX_K_OUTER) {
+                if (L.group != i_g || L.d1 != i_d1) i_g = L.group, i_d1 = L.d1, i_wait = -1;
                 const int need = G.out_end[closure_r0_diag(G, L)];
-                if (need > out_wait) {
+                if (need > i_wait) {
                     wait[i] = outer_of.at({L.group, L.d1, need});
-                    out_wait = need;
+                    i_wait = need;
                 }
             }
         }
@@ -1215,13 +1284,13 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
     std::vector<int> hbO(n, -1), hbI(n, -1);
     int prevO = -1, prevI = -1;
     for (int i = 0; i < n; ++i) {
-        const bool outer = !one && S.launches[i].kind == BPMAX_K_OUTER;
+        const bool outer = on_o(i);
         int o = -1, in = -1;
         auto absorb = [&](int j) {
             if (j < 0) return;
             o = std::max(o, hbO[j]);
             in = std::max(in, hbI[j]);
-            if (!one && S.launches[j].kind == BPMAX_K_OUTER) o = std::max(o, j);
+            if (on_o(j)) o = std::max(o, j);
             else in = std::max(in, j);
         };
         absorb(outer ? prevO : prevI);
@@ -1231,11 +1300,15 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
         hbI[i] = in;
         (outer ? prevO : prevI) = i;
     }
-    auto before = [&](int a, int b) {
-        return (!one && S.launches[a].kind == BPMAX_K_OUTER) ? a <= hbO[b] : a <= hbI[b];
+    auto before = [&](int a, int b) { return on_o(a) ? a <= hbO[b] : a <= hbI[b]; };
+    auto r0_before = [&](const std::pair<int, int> &r, int b) {
+        return (r.first < 0 || before(r.first, b)) && (r.second < 0 || before(r.second, b));
     };
     using Key = std::tuple<int, int, int, int, int>;
-    std::map<Key, int> closed, ext_at, r0_last;
+    std::map<Key, int> closed, ext_at;
+    // last R0 launch writing each storage tile, per stream (R0 stream, closure stream): the
+    // earlier ones on a stream are ordered before it
+    std::map<Key, std::pair<int, int>> r0_last;
     std::vector<int> s2_last(S.groups.size(), -1);
     for (int i = 0; i < n; ++i) {
         const Launch &L = S.launches[i];
@@ -1243,7 +1316,10 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
         for (int k = L.first; k < L.first + L.count; ++k) {
             const WorkItem &w = S.items[k];
             const Key key{w.pair, w.i1, w.j1, w.I, w.J};
-            if (L.kind == BPMAX_K_OUTER) r0_last[key] = std::max(r0_last.count(key) ? r0_last[key] : -1, i);
+            if (L.kind == BPMAX_K_OUTER) {
+                auto &r = r0_last.emplace(key, std::make_pair(-1, -1)).first->second;
+                (on_o(i) ? r.first : r.second) = i;
+            }
             else if (L.d1 >= 0 && L.kind == BPMAX_K_INNER_INTRA && wi_type(w.flags) != TASK_EXT) {
                 if (closed.count(key)) return fail("a tile is closed twice");
                 closed[key] = i;
@@ -1290,6 +1366,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                             return fail("a cross-tile step is never run");
                     }
     long long nchk = 0;
+    std::map<Key, std::vector<char>> r0_cov; // (k1, K) steps run per R0 output tile
     for (int i = 0; i < n; ++i) {
         const Launch &L = S.launches[i];
         for (int k = L.first; k < L.first + L.count; ++k) {
@@ -1301,9 +1378,24 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                      w.I, w.J);
             if (L.kind == BPMAX_K_OUTER) {
                 if (w.j1 == w.i1 && !before(s2_last[L.group], i)) return fail(std::string("R0 before S2") + where);
-                // closure goes by logical diagonal, and the closure order is checked below,
-                // so the last-closed tile of the square read suffices
-                for (int k1 = w.i1; k1 < w.j1; ++k1)
+                if ((w.flags & WI_EPILOGUE) && w.j1 - w.i1 >= 4) { // P1 reads F(i1+1, j1-1)
+                    const Key src{w.pair, w.i1 + 1, w.j1 - 1, w.I * rs, std::min((w.J + 1) * rs, p.nblr) - 1};
+                    if (!before(closed.at(src), i)) return fail(std::string("R0 epilogue before its P1 term") + where);
+                }
+                // the item's k1 values; closure goes by logical diagonal, and the closure order
+                // is checked below, so the last-closed tile of the square read suffices
+                const int kcnt = (w.flags >> WI_KCNT_SHIFT) & 255, nK = kcnt ? kcnt : w.J - w.I + 1;
+                std::vector<int> k1s;
+                const int Kb = w.I + ((w.flags >> WI_KOFF_SHIFT) & 255);
+                auto &cov = r0_cov[Key{w.pair, w.i1, w.j1, w.I, w.J}];
+                if (cov.empty()) cov.assign((size_t)(w.j1 - w.i1) * (w.J - w.I + 1), 0);
+                for (int st = w.step_begin; st < w.step_end; ++st) {
+                    const int k1 = outer_k1(w, st, nK), K = Kb + st % nK;
+                    if (k1 < w.i1 || k1 >= w.j1 || K > w.J) return fail(std::string("R0 step outside the triangle") + where);
+                    cov[(size_t)(k1 - w.i1) * (w.J - w.I + 1) + (K - w.I)] = 1;
+                    if (k1s.empty() || k1s.back() != k1) k1s.push_back(k1);
+                }
+                for (int k1 : k1s)
                     for (int side = 0; side < 2; ++side) {
                         const int a = side ? k1 + 1 : w.i1, b = side ? w.j1 : k1;
                         const Key src{w.pair, a, b, w.I * rs, std::min((w.J + 1) * rs, p.nblr) - 1};
@@ -1314,7 +1406,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                 // cross-tile steps: step s reads X tile (I, I+s) (R1, s < D) or
                 // (I+1+s-D, J) (R2); it writes the tile, after its R0
                 const auto r0 = r0_last.find(Key{w.pair, w.i1, w.j1, w.I / rs, w.J / rs});
-                if (r0 == r0_last.end() || !before(r0->second, i)) return fail(std::string("closure before R0") + where);
+                if (r0 == r0_last.end() || !r0_before(r0->second, i)) return fail(std::string("closure before R0") + where);
                 const int D = w.J - w.I;
                 for (int st = w.step_begin; st < w.step_end; ++st) {
                     const Key src = st < D ? Key{w.pair, w.i1, w.j1, w.I, w.I + st}
@@ -1324,7 +1416,7 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                 }
             } else if (L.d1 >= 0) {
                 const auto r0 = r0_last.find(Key{w.pair, w.i1, w.j1, w.I / rs, w.J / rs});
-                if (r0 == r0_last.end() || !before(r0->second, i)) return fail(std::string("closure before R0") + where);
+                if (r0 == r0_last.end() || !r0_before(r0->second, i)) return fail(std::string("closure before R0") + where);
                 for (int Y = w.I; Y < w.J; ++Y)
                     if (!before(closed.at(Key{w.pair, w.i1, w.j1, w.I, Y}), i))
                         return fail(std::string("closure before the tile left of it") + where);
@@ -1340,6 +1432,15 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
             }
         }
     }
+    for (int p = 0; p < npairs; ++p)
+        for (int i1 = 0; i1 < P[p].M; ++i1)
+            for (int j1 = i1 + 1; j1 < P[p].M; ++j1)
+                for (int I = 0; I < P[p].nb; ++I)
+                    for (int J = I; J < P[p].nb; ++J) {
+                        const auto c = r0_cov.find(Key{p, i1, j1, I, J});
+                        if (c == r0_cov.end() || std::count(c->second.begin(), c->second.end(), 0))
+                            return fail("an R0 step of a tile is never run");
+                    }
     if (checked) *checked = nchk;
     return BPMAX_OK;
 }
@@ -1484,7 +1585,8 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
 
     // The schedule is rebuilt only when the shapes or the schedule settings change.
     std::vector<long long> key = {sched,           ctx->task_chunk,   eff.outer_split, eff.outer_items,
-                                  eff.outer_kgroup, eff.closure_ahead, eff.tile,        npairs};
+                                  eff.outer_kgroup, eff.closure_ahead, eff.tile,        npairs,
+                                  ctx->r0_ahead};
     // (intra_wide changes kernel choices, not the schedule: the graph key carries it)
     for (auto &d : P) {
         key.push_back(d.M);
@@ -1509,7 +1611,7 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         } else {
             // one stream: a single R0 launch per outer diagonal (nothing to overlap with)
             build_schedule(P, S, sched == BPMAX_SCHED_LAUNCH ? 1 : eff.outer_split, eff.outer_items,
-                           eff.outer_kgroup, eff.closure_ahead != 0);
+                           eff.outer_kgroup, eff.closure_ahead != 0, sched == BPMAX_SCHED_PIPE && ctx->r0_ahead);
         }
         ctx->sched_key = key;
     }

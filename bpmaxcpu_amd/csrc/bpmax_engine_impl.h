@@ -27,6 +27,8 @@ struct Launch {
     int diag;            // storage block diagonal (outer) or logical block diagonal (inner)
     bool ahead = false;  // in-tile launch that also carries diagonal diag+1's early
                          // cross-tile steps (k_inner_closure_blk)
+    bool bulk = false;   // R0 launch of the k1 terms that read only the triangles two or more
+                         // outer diagonals below (runs one outer diagonal ahead)
 };
 
 // launch-schedule settings of one problem class (bpmax_tuning)
@@ -65,8 +67,11 @@ int choose_tile(int N);
 int pick_tile(int N, int tile);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
+// r0_ahead: the bulk of each outer diagonal's R0 (k1 terms reading only triangles two or
+// more diagonals below) is emitted one diagonal early, for the R0 stream, and the two edge
+// k1 terms with the closure (see enqueue_pipe); otherwise R0 of d1 is emitted whole.
 void build_schedule(const std::vector<PairDesc> &P, Schedule &S, int outer_split, int outer_items, int outer_kgroup,
-                    bool closure_ahead);
+                    bool closure_ahead, bool r0_ahead = false);
 // S2 of the pairs with sel[p] set, group g (tile size T), logical diagonals < nblmax.
 void emit_s2(const std::vector<PairDesc> &P, const std::vector<char> &sel, int g, int T, int nblmax, Schedule &S);
 // One step of the outer wavefront for group g: pair p on outer diagonal d1s[p] (< 0: not in
@@ -112,6 +117,7 @@ struct bpmax_ctx {
                                          // (0: one per storage diagonal)
     int outer_items = 2048;              // work items an R0 launch aims for (split-K granularity)
     int outer_kgroup = 1;                // K values per R0 work item (0: every K of the tile)
+    bool r0_ahead = true;                // pipelined schedule: bulk R0 one outer diagonal ahead
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
                                          // 1024-lane (latency) form, larger ones the blocked
                                          // (or, with intra_blocked = 0, the 512-lane) form

@@ -49,11 +49,19 @@ struct WorkItem {
 enum : int {
     WI_ATOMIC = 1,     // several items write this tile: combine with atomic max
     WI_EPILOGUE = 2,   // this item also adds init/P1/R3/R4 (outer phase)
+    WI_EDGE = 4,       // outer items: the steps of k1 = i1, then of k1 = j1-1 (the two terms
+                       // that read the triangles one outer diagonal below)
     WI_TYPE_SHIFT = 4, // task type (scheduler), 4 bits
     WI_KOFF_SHIFT = 8, // outer items: first K of the item's K range minus I, 8 bits
     WI_KCNT_SHIFT = 16, // outer items: number of K values of the range (0: all of I..J)
 };
 __host__ __device__ inline int wi_type(int flags) { return (flags >> WI_TYPE_SHIFT) & 15; }
+// k1 of step s of an outer item whose steps run nK K values per k1
+__host__ __device__ inline int outer_k1(const WorkItem &it, int s, int nK)
+{
+    const int q = s / nK;
+    return (it.flags & WI_EDGE) ? (q ? it.j1 - 1 : it.i1) : it.i1 + q;
+}
 
 enum : int { TASK_S1 = 1, TASK_S2EXT, TASK_S2INTRA, TASK_OUTER, TASK_EXT, TASK_INTRA };
 

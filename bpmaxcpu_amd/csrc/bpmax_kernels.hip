@@ -191,7 +191,7 @@ __device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__re
     };
     const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
     auto issue = [&](int h) {
-        const int s = h >> 1, kh = h & 1, k1 = i1 + s / nK, K = Kb + s % nK;
+        const int s = h >> 1, kh = h & 1, k1 = outer_k1(it, s, nK), K = Kb + s % nK;
         float *st = smem + (h & 1) * DlStage::FLOATS;
         const float *Bf = tri_at(arena, P, k1 + 1, j1);
         dl_stage_A(st, tri_at(arena, P, i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), kh, wv, l);
@@ -282,7 +282,7 @@ __device__ void outer_body_rs(const PairDesc &P, const WorkItem &it, float *__re
     // that the step's wait for them is the wait for the tiles, not one for the next prefetch
     float s1a_f = NEG_INF, s1b_f = NEG_INF;
     auto fetch = [&](int s) {
-        const int k1 = i1 + s / nK, K = Kb + s % nK;
+        const int k1 = outer_k1(it, s, nK), K = Kb + s % nK;
         const float *Bt = tri_at(arena, P, k1 + 1, j1);
         fetch_A<T, T>(fa, tri_at(arena, P, i1, k1), nb, I, K, 0);
         fetch_B<T, T>(fb, Bt, nb, K, J, 1);

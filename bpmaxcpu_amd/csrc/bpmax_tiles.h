@@ -396,13 +396,16 @@ __device__ __forceinline__ unsigned lds_offset(const void *p)
 // orders every later LDS read of the wave behind these loads (it cannot tell the two stages
 // apart), which serialises the prefetch with the product.  These loads have no register
 // result; a stage is published by vm_wait_all() and a barrier.
+// M0 is reserved to the compiler, which ignores it in a clobber list: these asm blocks are
+// correct only while no compiler-generated code of the file uses M0 (tests/test_abi.py
+// compiles the kernels to assembly and checks that every use of M0 is one of these pairs).
 __device__ __forceinline__ void g2lds16(const void *g, const void *lds)
 {
-    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(lds_offset(lds)) : "memory", "m0");
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(g), "s"(lds_offset(lds)) : "memory");
 }
 __device__ __forceinline__ void g2lds4(const void *g, const void *lds)
 {
-    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" : : "v"(g), "s"(lds_offset(lds)) : "memory", "m0");
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" : : "v"(g), "s"(lds_offset(lds)) : "memory");
 }
 __device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" : : : "memory"); }
 

@@ -188,3 +188,28 @@ def test_bench_tree6_addresses_every_level():
                     l5 = ctypes.cast(int(l4[d]), U)
                     for e in range(Nt + 1):
                         assert int(l5[e]) == lin.ctypes.data + 4 * ((((a * M + b) * Ns + c) * Ns + d) * sec + e * Nt)
+
+
+def test_m0_is_used_only_by_the_lds_stage_loads(tmp_path):
+    """The stage loads set M0 in inline assembly, which the compiler cannot be told about (M0
+    is reserved): compiled to gfx950 assembly, the kernels must touch M0 nowhere else, and
+    every write of it must be followed by the LDS load that reads it."""
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    csrc = os.path.join(os.path.dirname(B.__file__), "csrc")
+    out = tmp_path / "kernels.s"
+    subprocess.run([hipcc, "-O3", "-std=c++17", "-I", os.path.join(csrc, "..", "..", "include"), "-I", csrc,
+                    "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                    "-fno-honor-nans", "--cuda-device-only", "-S", os.path.join(csrc, "bpmax_kernels.hip"),
+                    "-o", str(out)], check=True, capture_output=True)
+    code = [ln.split(";")[0].strip() for ln in out.read_text().splitlines()]
+    code = [ln for ln in code if ln and not ln.startswith(".") and not ln.endswith(":")]
+    writes = 0
+    for i, ln in enumerate(code):
+        if "m0" not in ln.replace(",", " ").split():
+            continue
+        assert ln.startswith("s_mov_b32 m0,"), ln
+        assert code[i + 1].startswith("global_load_lds_dword"), code[i + 1]
+        writes += 1
+    assert writes >= 4
