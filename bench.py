@@ -176,9 +176,11 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
     achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
     traffic = None
     tpath = os.path.join(REPO, "profiles", "k_outer_64x512_traffic.json")
-    if (M, N) == (64, 512) and os.path.exists(tpath):
+    if (M, N) == (64, 512) and os.path.exists(tpath) and o["launches"]:
+        # rocprofv3 FETCH_SIZE + WRITE_SIZE of one solve's k_outer launches (copy-calibrated),
+        # spread over this run's launches per solve (the tuned R0 split changes their number)
         with open(tpath) as fh:
-            traffic = json.load(fh)["bytes_per_launch"]  # rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated
+            traffic = int(round(json.load(fh)["bytes_per_solve"] * steps / o["launches"]))
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_outer<64>: R0 double max-plus + init/P1/R3/R4 (2 flops per cell update)",
