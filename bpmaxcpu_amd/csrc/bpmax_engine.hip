@@ -323,7 +323,7 @@ static int tune_pairs(bpmax_ctx *ctx, int npairs, const int *const *p1, const in
             cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, items, 0, w0, k0, a0});
             for (int split : {1, 2, 4, 0}) cands.push_back(Tuning{BPMAX_SCHED_PIPE, split, items, 0, w0, k0, a0});
         }
-        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 2, 2048, 1, w0, k0, a0});
+        cands.push_back(Tuning{BPMAX_SCHED_PIPE, 1, 4096, 1, w0, k0, a0});
     } else {
         cands.push_back(Tuning{BPMAX_SCHED_LAUNCH, 1, 2048, 0, w0, k0, a0});
         for (int items : {2048, 4096})

@@ -113,9 +113,9 @@ struct bpmax_ctx {
     char *meta_dev = nullptr;            // meta block (weights | descs | items) of the last run
     // pipelined schedule: second stream, dependency events, cached graph
     hipStream_t stream2 = nullptr;       // inner closure
-    int outer_split = 2;                 // R0 launches per outer diagonal, pipelined schedule
+    int outer_split = 1;                 // R0 launches per outer diagonal (per part), pipelined schedule
                                          // (0: one per storage diagonal)
-    int outer_items = 2048;              // work items an R0 launch aims for (split-K granularity)
+    int outer_items = 4096;              // work items an R0 launch aims for (split-K granularity)
     int outer_kgroup = 1;                // K values per R0 work item (0: every K of the tile)
     bool r0_ahead = true;                // pipelined schedule: bulk R0 one outer diagonal ahead
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
