@@ -31,8 +31,8 @@ def eng():
 
 
 @pytest.fixture(scope="module",
-                params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "pipe-t32", "launch-narrow", "launch-wide",
-                        "launch-ahead", "tasks"])
+                params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "pipe-t32", "pipe-r0whole", "launch-narrow",
+                        "launch-wide", "launch-ahead", "tasks"])
 def eng2(request):
     """An engine under each schedule:
     * pipe: the two-stream pipeline with the default settings (inner closure in one launch
@@ -46,14 +46,17 @@ def eng2(request):
       each, the last one ragged) or in one launch per diagonal (ahead);
     * pipe-t32: the default pipeline with 32 x 32 storage tiles for every N (the tuner's
       tile candidate);
+    * pipe-r0whole: the pipeline with each outer diagonal's R0 run whole after the closure
+      of the diagonal below (instead of its bulk one diagonal ahead), in 2 ranges;
     * tasks: the dependency-tile scheduler."""
     push = request.param.endswith("push")
-    if push:  # read when the context is created
-        os.environ["BPMAX_INTRA_BLOCKED"] = "0"
+    env = {"BPMAX_INTRA_BLOCKED": "0"} if push else ({"BPMAX_R0_AHEAD": "0"} if request.param == "pipe-r0whole" else {})
+    os.environ.update(env)  # read when the context is created
     try:
         e = B.Engine(0)
     finally:
-        os.environ.pop("BPMAX_INTRA_BLOCKED", None)
+        for k in env:
+            os.environ.pop(k, None)
     sched = request.param.split("-")[0]
     ahead = request.param.endswith("ahead")
     if sched == "launch":
