@@ -299,26 +299,10 @@ __device__ void outer_body_rs(const PairDesc &P, const WorkItem &it, float *__re
     // The ranges are fixed per item here (the per-step balanced_k_range, which also serves
     // items of several K, timed 0.4 ms slower on a 64x512 solve of one-K items).
     constexpr int QS = Tc<T>::QS, H = QS / 2;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), qr = wv >> 1, qc = wv & 1;
-    int mb = 0, me = T, hw = -1, hb = 0, he = 0; // own range; helped wave and its slice
-    bool helped = false;
-    const bool balance = BAL && nK == 1 && (Kb == I || Kb == J) && it.step_begin < it.step_end;
-    if (balance) {
-        if (Kb == I && Kb == J) { // w0 [0,QS)  w1 [0,2QS) -> w1 [QS,2QS) + w2 [0,QS)  w3 [QS,2QS)
-            mb = wv == 3 || wv == 1 ? QS : 0;
-            me = wv == 0 ? QS : (wv == 2 ? 0 : T);
-            if (wv == 2) { hw = 1; hb = 0; he = QS; }
-            helped = wv == 1;
-        } else if (Kb == I) { // rows of qr = 1 need k >= QS: qr 0 [H,2QS), qr 1 [QS,2QS) + [0,H) of (0,qc)
-            mb = qr ? QS : H;
-            if (qr) { hw = qc; hb = 0; he = H; }
-            helped = qr == 0;
-        } else { // columns of qc = 0 need k < QS: qc 1 [0,2QS-H), qc 0 [0,QS) + [2QS-H,2QS) of (qr,1)
-            me = qc ? T - H : QS;
-            if (!qc) { hw = 2 * qr + 1; hb = T - H; he = T; }
-            helped = qc == 1;
-        }
-    }
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const OuterRanges g = outer_ranges<T, BAL>(it, nK, Kb, wv);
+    const int mb = g.mb, me = g.me, hw = g.hw, hb = g.hb, he = g.he;
+    const bool balance = g.balance;
     float acc2[R][R];
 #pragma unroll
     for (int i = 0; i < R; ++i)
@@ -363,43 +347,9 @@ __device__ void outer_body_rs(const PairDesc &P, const WorkItem &it, float *__re
         if (!own_pad) product<T, H>(acc, As, Bs, r0, c0, kb, ke < klim ? ke : klim);
         if (hw >= 0 && !help_pad) product<T, H>(acc2, As, Bs, hr0, hc0, hb, he < klim ? he : klim);
     }
-    if (balance) { // hand the slices over through the (now free) operand LDS
-        constexpr int RG = R * R * 64;
-        const int l = threadIdx.x % 64;
-        __syncthreads();
-        if (hw >= 0) {
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int j = 0; j < R; ++j) smem[hw * RG + i * R * 64 + l * R + j] = acc2[i][j];
-        }
-        __syncthreads();
-        if (helped) {
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int j = 0; j < R; ++j) acc[i][j] = fmaxf(acc[i][j], smem[wv * RG + i * R * 64 + l * R + j]);
-        }
-    }
+    if (balance) outer_merge<T>(g, acc, acc2, smem, wv); // through the (now free) operand LDS
 
-    if (it.flags & WI_EPILOGUE) {
-        const float *S2t = arena + P.S2 + toff;
-        const int *a = codes + P.a, *b = codes + P.b;
-        const float s1 = S1[i1 * M + j1];
-        const int d1 = j1 - i1;
-        const float w1 = d1 >= 4 ? w[5 * a[i1] + a[j1]] : 0.f;
-        const float *Pt = d1 >= 4 ? tri_at(arena, P, i1 + 1, j1 - 1) + toff : S2t;
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int lr = r0 + i, lc = c0 + j;
-                float v = s1 + S2t[lr * T + lc];
-                if (d1 == 0 && I * T + lr == J * T + lc) v = w[25 + 5 * a[i1] + b[I * T + lr]]; // inter base
-                if (d1 >= 4) v = fmaxf(v, Pt[lr * T + lc] + w1);
-                acc[i][j] = fmaxf(acc[i][j], v);
-            }
-    }
+    if (it.flags & WI_EPILOGUE) outer_epilogue<T>(P, i1, j1, toff, arena, codes, w, acc, r0, c0, I, J);
     store_acc<T, T>(tri_at(arena, P, i1, j1) + toff, acc, r0, c0, (it.flags & WI_ATOMIC) != 0);
 }
 
