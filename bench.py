@@ -31,6 +31,9 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
+# launch schedules picked by the autotuner on an MI355X for the bench's classes
+# (python -m bpmaxcpu_amd.autotune); --retune tunes them again instead
+TUNING_TABLE = os.path.join(REPO, "bpmaxcpu_amd", "tuning_mi355x.json")
 FP32_VALU_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector peak: 256 CUs x 2.4 GHz x 128 lanes x 2 (FMA)
 VALU_ISSUE_PEAK = 2 * 256 * 2.4e9  # wave64 FP32 VALU instructions per second (2 per clock per CU)
 
@@ -126,14 +129,17 @@ def parse_schedule(text):
             "tile": int(f[7]) if len(f) > 7 else 0}
 
 
-def pair_leg(B, eng, M, N, steps, warmup, schedule=None):
+def pair_leg(B, eng, M, N, steps, warmup, schedule=None, retune=False):
     s1, s2 = workload(M, N)
     cells = B.cell_updates(M, N)
     t0 = time.perf_counter()
+    tuned = None if retune else eng.get_tuning(M, N, 1)
     if schedule:  # a fixed launch schedule (profiling runs: the same launches every run)
         tuned = parse_schedule(schedule)
         eng.set_tuning(M, N, 1, tuned)
         log(f"fixed schedule {tuned}")
+    elif tuned is not None:  # the class's entry in the shipped tuning table: the same launches every run
+        log(f"tuning table entry for {M}x{N}: {tuned}")
     else:
         tuned = eng.tune(M, N, 1, reps=2)  # launch schedule of this problem class (setup, untimed)
         log(f"autotuned {M}x{N} in {time.perf_counter() - t0:.1f} s: {tuned}")
@@ -293,14 +299,19 @@ def dropin_leg(B, eng, M, N, s1, s2, N_tile=48):
                                "bytes": int(M * (M + 1) // 2 * (N * (N + 1) // 2) * 4)}}
 
 
-def batched_leg(B, eng, reps=2):
+def batched_leg(B, eng, reps=2, retune=False):
     s1, s2 = mixed_pairs()
     n = len(s1)
     cells = sum(B.cell_updates(len(a), len(b)) for a, b in zip(s1, s2))
-    # the launch schedule of the batch class, tuned on this batch (setup, untimed)
-    t0 = time.perf_counter()
-    tuned = eng.tune_batch(s1, s2, reps=1)
-    log(f"autotuned the batch class in {time.perf_counter() - t0:.1f} s: {tuned}")
+    # the launch schedule of the batch class: the shipped table's entry (tuned on this batch),
+    # or tuned on this batch now (setup, untimed)
+    tuned = None if retune else eng.get_tuning(max(len(a) for a in s1), max(len(b) for b in s2), n)
+    if tuned is not None:
+        log(f"tuning table entry for the batch class: {tuned}")
+    else:
+        t0 = time.perf_counter()
+        tuned = eng.tune_batch(s1, s2, reps=1)
+        log(f"autotuned the batch class in {time.perf_counter() - t0:.1f} s: {tuned}")
     log(f"batched: {n} mixed pairs, {cells / 1e12:.2f} T cells")
     eng.solve_batch(s1, s2)
     ts = []
@@ -465,7 +476,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and isolated-R0 legs")
     ap.add_argument("--schedule", default=None, metavar="S:SPLIT:ITEMS:GRAPHS:WIDE:KGROUP[:AHEAD]",
-                    help="fixed launch schedule of the pair leg instead of autotuning it, e.g. pipe:2:4096:0:64:2")
+                    help="fixed launch schedule of the pair leg instead of the tuning table's, e.g. pipe:2:4096:0:64:2")
+    ap.add_argument("--retune", action="store_true",
+                    help="autotune the pair and batch classes on this GPU instead of loading the shipped "
+                         "tuning table (bpmaxcpu_amd/tuning_mi355x.json)")
+    ap.add_argument("--write-tuning", default=None, metavar="PATH",
+                    help="only autotune the pair class (MxN) and the batched leg's class (on its own 256 "
+                         "pairs) on this GPU and write them as a tuning table to PATH")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (reads MIB, writes MIB): the "
@@ -508,12 +525,25 @@ def main():
     import bpmaxcpu_amd as B
     M, N = args.M, args.N
     eng = B.Engine(0)
-    res = pair_leg(B, eng, M, N, args.steps, args.warmup, args.schedule)
+    if args.write_tuning:
+        s1, s2 = mixed_pairs()
+        table = [{"M": M, "N": N, "pairs": 1, **eng.tune(M, N, 1, reps=3)},
+                 {"M": max(len(a) for a in s1), "N": max(len(b) for b in s2), "pairs": len(s1),
+                  **eng.tune_batch(s1, s2, reps=2)}]
+        with open(args.write_tuning, "w") as fh:
+            json.dump(table, fh, indent=1)
+        print(json.dumps(table), flush=True)
+        eng.close()
+        return
+    if not args.retune and os.path.exists(TUNING_TABLE):
+        from bpmaxcpu_amd.autotune import load_table
+        load_table(eng, TUNING_TABLE)
+    res = pair_leg(B, eng, M, N, args.steps, args.warmup, args.schedule, args.retune)
     batched = r0 = serving = dropin = None
     if not args.no_extra:
         s1d, s2d = workload(M, N)
         dropin = dropin_leg(B, eng, M, N, s1d, s2d)
-        batched = batched_leg(B, eng)
+        batched = batched_leg(B, eng, retune=args.retune)
         r0 = r0_leg(B, eng)
         serving = serving_leg(B, eng)
 
