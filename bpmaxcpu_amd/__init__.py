@@ -73,6 +73,8 @@ def load_library(path: str = LIB_PATH):
         "bpmax_tri_fill_random": (c_int, [c_void_p, c_int, c_int, ctypes.c_ulonglong, c_int, c_int, c_void_p]),
         "bpmax_device_alloc": (c_void_p, [c_void_p, c_size_t]),
         "bpmax_device_free": (c_int, [c_void_p, c_void_p]),
+        "bpmax_ctx_set_r0_half": (c_int, [c_void_p, c_int]),
+        "bpmax_ctx_r0_half_used": (c_int, [c_void_p]),
         "bpmax_ctx_set_timing": (c_int, [c_void_p, c_int]),
         "bpmax_ctx_set_timing_mask": (c_int, [c_void_p, ctypes.c_uint]),
         "bpmax_ctx_kernel_stats": (c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_longlong),
@@ -353,6 +355,14 @@ class Engine:
         if isinstance(schedule, str):
             schedule = SCHEDULES[schedule]
         self._check(self.L.bpmax_ctx_set_schedule(self.h, schedule, chunk))
+
+    def set_r0_half(self, on: bool):
+        """FP16 R0 operands where they are exact (default on; bpmax_ctx_set_r0_half)."""
+        self._check(self.L.bpmax_ctx_set_r0_half(self.h, 1 if on else 0))
+
+    def r0_half_used(self) -> bool:
+        """Whether the last solve ran R0 on FP16 operands (bpmax_ctx_r0_half_used)."""
+        return bool(self.L.bpmax_ctx_r0_half_used(self.h))
 
     def set_graphs(self, on: bool):
         """Capture and replay the pipelined schedule as a HIP graph (default off)."""

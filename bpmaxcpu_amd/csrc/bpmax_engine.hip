@@ -88,6 +88,26 @@ bool weights_exact(const float *intra, const float *inter, int M, int N, std::st
     return true;
 }
 
+// The same bound against FP16 (11 significant bits): every score of an M x N pair is an
+// integer multiple of 2^-p no larger than wmax * ((M+N)/2 + 1), so with that bound times 2^p
+// at most 2^11 every score, and every R0 sum (never above the final value of its cell), is an
+// exact FP16 number.  Weights that pass weights_exact only.
+bool weights_half_exact(const float *intra, const float *inter, int M, int N)
+{
+    int p = -1;
+    for (int cand = 0; cand <= 12 && p < 0; ++cand) {
+        bool ok = true;
+        for (int t = 0; t < 50 && ok; ++t) {
+            const double s = std::ldexp((double)(t < 25 ? intra[t] : inter[t - 25]), cand);
+            ok = (s == std::floor(s));
+        }
+        if (ok) p = cand;
+    }
+    float wmax = 0.f;
+    for (int t = 0; t < 25; ++t) wmax = std::max(wmax, std::max(intra[t], inter[t]));
+    return p >= 0 && (double)wmax * (((double)M + N) / 2.0 + 1.0) * std::ldexp(1.0, p) <= 2048.0;
+}
+
 } // namespace bpx
 
 // ----------------------------------------------------------------------------- ctx
@@ -156,6 +176,7 @@ extern "C" int bpmax_ctx_create(bpmax_ctx **out, int device)
     if (const char *e = getenv("BPMAX_R0_AHEAD")) c->r0_ahead = atoi(e) != 0;
     if (const char *e = getenv("BPMAX_CLOSURE_AHEAD")) c->closure_ahead = atoi(e) != 0;
     if (const char *e = getenv("BPMAX_INTRA_BLOCKED")) c->intra_blocked = atoi(e) != 0;
+    if (const char *e = getenv("BPMAX_R0_HALF")) c->r0_half = atoi(e) != 0;
     *out = c;
     return BPMAX_OK;
 }
@@ -234,6 +255,16 @@ extern "C" int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable)
     ctx->graphs = enable != 0;
     return BPMAX_OK;
 }
+
+extern "C" int bpmax_ctx_set_r0_half(bpmax_ctx *ctx, int enable)
+{
+    if (!ctx) return BPMAX_EINVAL;
+    ctx->r0_half = enable != 0;
+    return BPMAX_OK;
+}
+
+extern "C" int bpmax_ctx_r0_half_used(const bpmax_ctx *ctx) { return ctx && ctx->run_half ? 1 : 0; }
+
 
 // ------------------------------------------------------------------------- autotuning
 namespace bpx {
@@ -994,8 +1025,9 @@ int bpx_collect_timing(bpmax_ctx *ctx)
 // of more than kMaxItems work items (one workgroup each) goes out in pieces: the items of
 // one launch are independent, and a grid stays far below the 2^32 threads a dispatch holds.
 constexpr int kMaxItems = 1 << 22;
+// half: R0 launches read the FP16 mirrors (every pair of the run has one, ctx->run_half).
 static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDesc *dP, const WorkItem *dI,
-                      float *arena, const int *dcodes, const float *dw)
+                      float *arena, const int *dcodes, const float *dw, bool half)
 {
     if (L.count == 0) return BPMAX_OK;
     if (L.count > kMaxItems) {
@@ -1004,7 +1036,7 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
             part.first = L.first + f;
             part.count = std::min(kMaxItems, L.count - f);
             part.updates = f == 0 ? L.updates : 0;
-            const int rc = run_launch(ctx, st, part, dP, dI, arena, dcodes, dw);
+            const int rc = run_launch(ctx, st, part, dP, dI, arena, dcodes, dw, half);
             if (rc != BPMAX_OK) return rc;
         }
         return BPMAX_OK;
@@ -1013,7 +1045,7 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
     switch (L.kind) {
     case BPMAX_K_OUTER:
         return timed(ctx, st, L.kind, L.updates,
-                     [&] { return launch_outer(st, L.T, dP, items, L.count, arena, dcodes, dw); });
+                     [&] { return launch_outer(st, L.T, dP, items, L.count, arena, dcodes, dw, half); });
     case BPMAX_K_INNER_EXT:
         return timed(ctx, st, L.kind, L.updates,
                      [&] { return launch_inner_ext(st, L.T, L.single_strand, dP, items, L.count, arena); });
@@ -1031,7 +1063,7 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
 int bpx_run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDesc *dP, const WorkItem *dI,
                    float *arena, const int *dcodes, const float *dw)
 {
-    return run_launch(ctx, st, L, dP, dI, arena, dcodes, dw);
+    return run_launch(ctx, st, L, dP, dI, arena, dcodes, dw, false); // server pairs have no FP16 mirror
 }
 
 // Dependency events of the pipelined schedule: fill, S2 done, end, then per T group one
@@ -1101,7 +1133,8 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
     for (const Launch &L : S.launches)
         if (L.d1 >= 0 && on_r0_stream(L) && L.d1 - lag(L) >= 0) waited[L.group][in_need(L)] = 1;
 
-    if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0, [&] { return launch_fill(sO, arena, fl, -INFINITY); })) != BPMAX_OK)
+    if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0,
+                    [&] { return launch_fill2(sO, arena, fl, -INFINITY, ctx->fill_half, 0xFC00FC00u); })) != BPMAX_OK)
         return rc;
     HIPCHK(ctx, hipEventRecord(ev_fill, sO));
     HIPCHK(ctx, hipStreamWaitEvent(sI, ev_fill, 0));
@@ -1110,7 +1143,7 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
         BPMAX_OK)
         return rc;
     for (const Launch &L : S.launches)
-        if (L.d1 < 0 && (rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+        if (L.d1 < 0 && (rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
     HIPCHK(ctx, hipEventRecord(ev_s2, sI));
 
     int o_g = -1, o_src = -2, o_wait = -1; // waits issued on sO for the closure of (group, o_src)
@@ -1131,7 +1164,7 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
                     o_wait = need;
                 }
             }
-            if ((rc = run_launch(ctx, sO, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+            if ((rc = run_launch(ctx, sO, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
             HIPCHK(ctx, hipEventRecord(ev_out(L.group, L.d1, L.diag), sO));
         } else {
             if (L.group != i_g || L.d1 != i_d1) i_g = L.group, i_d1 = L.d1, i_wait = -1;
@@ -1140,7 +1173,7 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
                 HIPCHK(ctx, hipStreamWaitEvent(sI, ev_out(L.group, L.d1, need), 0));
                 i_wait = need;
             }
-            if ((rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+            if ((rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
             if (L.kind == BPMAX_K_INNER_INTRA && waited[L.group][L.diag])
                 HIPCHK(ctx, hipEventRecord(ev_in(L.group, L.d1, L.diag), sI));
         }
@@ -1163,6 +1196,8 @@ static int launch_pipe_graph(bpmax_ctx *ctx, const std::vector<long long> &key, 
     }
     gk.push_back((long long)(uintptr_t)arena);
     gk.push_back(ctx->eff_intra_wide); // kernel choices that vary between runs of one shape
+    gk.push_back(ctx->run_half);
+    gk.push_back(ctx->fill_half);
     gk.push_back(fl);
     if (!ctx->graph || gk != ctx->graph_key) {
         if (ctx->graph) HIPCHK(ctx, hipGraphExecDestroy(ctx->graph));
@@ -1528,12 +1563,25 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         il += al((long long)d.nb * d.T);
         P[p] = d;
     }
+    // FP16 mirrors of F after all FP32 tables, when every pair is on 64 x 64 tiles and exact in
+    // FP16 (the R0 launches then read them; the no-allocation mode keeps its scratch for FP32)
+    bool half = ctx->r0_half && !ctx->carve && eff.schedule != BPMAX_SCHED_TASKS && ctx->schedule != BPMAX_SCHED_TASKS;
+    for (int p = 0; p < npairs && half; ++p)
+        half = P[p].T == 64 && weights_half_exact(wt, wt + 25, P[p].M, P[p].N);
+    long long fh = 0;
+    if (half)
+        for (auto &d : P) {
+            d.Fh = fl + fh;
+            fh += al((tri_ll(d.M) * d.tstride + 1) / 2);
+        }
+    ctx->run_half = half;
+    ctx->fill_half = fh;
     int rc;
     char *codes_dev = nullptr, *meta_dev = nullptr;
     size_t meta_room = 0;
     auto al256 = [](size_t x) { return (x + 255) / 256 * 256; };
     if (ctx->carve) { // no-allocation mode: arena | codes | meta carved from the caller's region
-        const size_t ab = al256((size_t)fl * sizeof(float)), cb = al256((size_t)il * sizeof(int));
+        const size_t ab = al256((size_t)(fl + fh) * sizeof(float)), cb = al256((size_t)il * sizeof(int));
         if (ab + cb + 256 > ctx->carve_bytes) {
             ctx->err = "the pair needs " + std::to_string(ab + cb) + " bytes of device scratch for its tables, more "
                        "than the " + std::to_string(ctx->carve_bytes) + " available";
@@ -1546,12 +1594,12 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         meta_room = ctx->carve_bytes - ab - cb;
     } else {
         if (ctx->arena_ext) {
-            if ((size_t)fl * sizeof(float) > ctx->arena_ext_bytes) {
-                ctx->err = "pairs need " + std::to_string((size_t)fl * sizeof(float)) +
+            if ((size_t)(fl + fh) * sizeof(float) > ctx->arena_ext_bytes) {
+                ctx->err = "pairs need " + std::to_string((size_t)(fl + fh) * sizeof(float)) +
                            " arena bytes, the slice holds " + std::to_string(ctx->arena_ext_bytes);
                 return BPMAX_ENOMEM;
             }
-        } else if ((rc = bpx_grow(ctx, ctx->arena, (size_t)fl * sizeof(float))) != BPMAX_OK) {
+        } else if ((rc = bpx_grow(ctx, ctx->arena, (size_t)(fl + fh) * sizeof(float))) != BPMAX_OK) {
             return rc;
         }
         if ((rc = bpx_grow(ctx, ctx->codes, (size_t)il * sizeof(int))) != BPMAX_OK) return rc;
@@ -1699,14 +1747,15 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
             rc = enqueue_pipe(ctx, npairs, S, dP, dI, arena, fl, dcodes, dw);
         if (rc != BPMAX_OK) return rc;
     } else {
-        if ((rc = timed(ctx, st, BPMAX_K_FILL, 0, [&] { return launch_fill(st, arena, fl, -INFINITY); })) !=
+        if ((rc = timed(ctx, st, BPMAX_K_FILL, 0,
+                        [&] { return launch_fill2(st, arena, fl, -INFINITY, ctx->fill_half, 0xFC00FC00u); })) !=
             BPMAX_OK)
             return rc;
         if ((rc = timed(ctx, st, BPMAX_K_S1, 0, [&] { return launch_s1(st, dP, npairs, arena, dcodes, dw); })) !=
             BPMAX_OK)
             return rc;
         for (const Launch &L : S.launches)
-            if ((rc = run_launch(ctx, st, L, dP, dI, arena, dcodes, dw)) != BPMAX_OK) return rc;
+            if ((rc = run_launch(ctx, st, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
     }
     // the host vectors (codes, staging) are context members, alive until the next run
     if (!sync) return BPMAX_OK;

@@ -67,6 +67,7 @@ int choose_tile(int N);
 int pick_tile(int N, int tile);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
+bool weights_half_exact(const float *intra, const float *inter, int M, int N);
 // r0_ahead: the bulk of each outer diagonal's R0 (k1 terms reading only triangles two or
 // more diagonals below) is emitted one diagonal early, for the R0 stream, and the two edge
 // k1 terms with the closure (see enqueue_pipe); otherwise R0 of d1 is emitted whole.
@@ -118,6 +119,9 @@ struct bpmax_ctx {
     int outer_items = 4096;              // work items an R0 launch aims for (split-K granularity)
     int outer_kgroup = 1;                // K values per R0 work item (0: every K of the tile)
     bool r0_ahead = true;                // pipelined schedule: bulk R0 one outer diagonal ahead
+    bool r0_half = true;                 // R0 on FP16 operands where that is exact (setting)
+    bool run_half = false;               // the current run's R0 launches use the FP16 mirrors
+    long long fill_half = 0;             // words of FP16 mirror after the run's FP32 tables
     int intra_wide = 256;                // in-tile launches of at most this many tiles use the
                                          // 1024-lane (latency) form, larger ones the blocked
                                          // (or, with intra_blocked = 0, the 512-lane) form

@@ -10,6 +10,9 @@
 //     reversed seq2).
 //   * S1: dense M x M (strand-1 single-strand table).
 //   * codes: seq1[M] and b[nb*T] = reversed seq2, padded with code 4 (N).
+//   * Fh (optional): F again in FP16, the R0 kernel's operand copy when every score of the
+//     pair is an integer multiple of 2^-p below 2^(11-p) (exact in FP16); cells that the
+//     closure never writes hold FP16 -inf.
 #pragma once
 #include <cstdint>
 
@@ -29,6 +32,8 @@ struct PairDesc {
     long long tstride; // floats per outer triangle = tri(nb)*T*T
     long long S2;      // float offset of the tiled S2 triangle
     long long S1;      // float offset of dense S1
+    long long Fh = -1; // float offset of the FP16 mirror of F (same tiled layout, one half
+                       // per cell; written when the inner closure finalizes a tile), or -1
     long long a;       // int offset of seq1 codes
     long long b;       // int offset of reversed, padded seq2 codes
     long long flag;    // int offset of the pair's dependency counters (task scheduler):

@@ -253,6 +253,129 @@ __device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__re
     else store_acc<T, T>(dst, acc, r0, c0, false);
 }
 
+// FP16 form of outer_body_dl (pairs whose scores FP16 holds exactly, PairDesc::Fh >= 0):
+// the R0 operands are staged from the FP16 mirror of F (HStage: 8.6 KB per half instead of
+// 16.5) and multiplied on packed halves (h_product: 4 cell updates per 3 VALU instructions);
+// R3/R4 read the same staged tiles.  The lane's 4 x 4 cells are held as 4 x 2 packed pairs
+// and turned back into FP32 (exactly) for the epilogue and the store, which are
+// outer_body_dl's.
+template <bool BAL>
+__device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
+                             const int *__restrict__ codes, const float *__restrict__ w, char *smem)
+{
+    constexpr int T = 64, R = 4, QS = Tc<T>::QS, H = QS / 2;
+    int r0, c0;
+    lane_block<T>(r0, c0);
+    const int i1 = it.i1, j1 = it.j1, I = it.I, J = it.J, nb = P.nb, M = P.M;
+    const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
+    const float *S1 = arena + P.S1;
+    const h16bits *Fh = (const h16bits *)(arena + P.Fh);
+    auto tri_h = [&](int a, int b) { return Fh + utri_ll(a, b, M) * P.tstride; };
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
+    const int kcnt = (it.flags >> WI_KCNT_SHIFT) & 255;
+    const int nK = kcnt ? kcnt : J - I + 1, Kb = I + ((it.flags >> WI_KOFF_SHIFT) & 255);
+    unsigned acc[R][2], acc2[R][2];
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc[i][0] = acc[i][1] = acc2[i][0] = acc2[i][1] = 0xFC00FC00u;
+    const OuterRanges g = outer_ranges<T, BAL>(it, nK, Kb, wv);
+    int hr0 = 0, hc0 = 0;
+    if (g.hw >= 0) lane_block<T>(hr0, hc0, g.hw * 64 + l);
+    const HRows rows = h_rows(r0), hrows = h_rows(hr0);
+    // padding (N not a multiple of T): see outer_body_rs
+    const int kreal = (P.N - (nb - 1) * T + H - 1) / H * H;
+    auto quad_pad = [&](int x) {
+        return (I == nb - 1 && I * T + (x >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (x & 1) * QS >= P.N);
+    };
+    const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
+    auto issue = [&](int h) {
+        const int s = h >> 1, kh = h & 1, k1 = outer_k1(it, s, nK), K = Kb + s % nK;
+        char *st = smem + (h & 1) * HStage::BYTES;
+        const h16bits *Bf = tri_h(k1 + 1, j1);
+        h_stage_A(st, tri_h(i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), kh, wv, l);
+        h_stage_B32(st, Bf + utri_ll(K, J, nb) * (long long)(T * T), kh, wv, l);
+        if (wv == 0 && l < 8) { // B row 32 of the half: row 32 of tile K, or row 0 of tile K+1 (-inf past J)
+            const h16bits *src = kh == 0 ? Bf + utri_ll(K, J, nb) * (long long)(T * T) + 32 * T
+                                         : (K + 1 <= J ? Bf + utri_ll(K + 1, J, nb) * (long long)(T * T)
+                                                       : (const h16bits *)kNegInfRowH);
+            g2lds16(src + 8 * l, st + HStage::B + 32 * 128);
+        }
+        if (kh == 0 && l == 0) {
+            if (wv == 1 && K == I) g2lds4(S1 + i1 * M + k1, st + HStage::S1);
+            if (wv == 2 && K == J) g2lds4(S1 + (k1 + 1) * M + j1, st + HStage::S1 + 4);
+        }
+    };
+    const int h0 = 2 * it.step_begin, h1 = 2 * it.step_end;
+    if (h0 < h1) issue(h0);
+    unsigned s1a = 0xFC00FC00u, s1b = 0xFC00FC00u;
+    for (int h = h0; h < h1; ++h) {
+        const int s = h >> 1, kh = h & 1, K = Kb + s % nK;
+        const char *st = smem + (h & 1) * HStage::BYTES;
+        vm_wait_all();
+        __syncthreads();
+        if (kh == 0) { // read before the next loads are issued into the other stage
+            const float a = *(const float *)(st + HStage::S1), b = *(const float *)(st + HStage::S1 + 4);
+            s1a = pk_pack(a, a);
+            s1b = pk_pack(b, b);
+        }
+        if (h + 1 < h1) issue(h + 1); // in flight while this half computes
+        if (K == J && kh == (wv & 1)) { // R4: A(r, c) = F(i1,k1)[I*T + r][J*T + c], columns c0 .. c0+3
+            const int c = (c0 - 32 * kh) >> 3, off = 2 * (c0 & 7);
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const uint2 v = *(const uint2 *)(st + rows.base[i] + 16 * (c ^ rows.sw[i]) + off);
+                const unsigned t0 = pk_add(v.x, s1b), t1 = pk_add(v.y, s1b);
+                pk_max3_into(acc[i][0], t0, t0);
+                pk_max3_into(acc[i][1], t1, t1);
+            }
+        }
+        if (K == I && kh == (wv >> 1)) { // R3: B row (r - 32kh) = F(k1+1,j1)[I*T + r][J*T + c]
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const uint2 v = *(const uint2 *)(st + HStage::B + (r0 + i - 32 * kh) * 128 + 2 * c0);
+                const unsigned t0 = pk_add(s1a, v.x), t1 = pk_add(s1a, v.y);
+                pk_max3_into(acc[i][0], t0, t0);
+                pk_max3_into(acc[i][1], t1, t1);
+            }
+        }
+        int kb = g.mb, ke = g.me;
+        if (!g.balance) quad_k_range<T>(K == I, K == J, false, kb, ke);
+        const int klim = K == nb - 1 ? kreal : T;
+        if (!own_pad) h_product<1>(acc, st, rows, c0, kh, kb, ke < klim ? ke : klim);
+        if (g.hw >= 0 && !help_pad) h_product<1>(acc2, st, hrows, hc0, kh, g.hb, g.he < klim ? g.he : klim);
+    }
+    if (g.balance) { // helpers hand their slices to the helped waves (as outer_merge, packed)
+        unsigned *x = (unsigned *)smem;
+        __syncthreads();
+        if (g.hw >= 0) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                x[g.hw * 512 + i * 128 + 2 * l] = acc2[i][0];
+                x[g.hw * 512 + i * 128 + 2 * l + 1] = acc2[i][1];
+            }
+        }
+        __syncthreads();
+        if (g.helped) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                pk_max3_into(acc[i][0], x[wv * 512 + i * 128 + 2 * l], acc[i][0]);
+                pk_max3_into(acc[i][1], x[wv * 512 + i * 128 + 2 * l + 1], acc[i][1]);
+            }
+        }
+    }
+    float acc32[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        acc32[i][0] = pk_lo(acc[i][0]);
+        acc32[i][1] = pk_hi(acc[i][0]);
+        acc32[i][2] = pk_lo(acc[i][1]);
+        acc32[i][3] = pk_hi(acc[i][1]);
+    }
+    if (it.flags & WI_EPILOGUE) outer_epilogue<T>(P, i1, j1, toff, arena, codes, w, acc32, r0, c0, I, J);
+    float *dst = tri_at(arena, P, i1, j1) + toff;
+    if (it.flags & WI_ATOMIC) store_rows_atomic(dst, acc32, r0, c0, (float *)smem, wv, l);
+    else store_acc<T, T>(dst, acc32, r0, c0, false);
+}
+
 // BAL: balance the waves' k ranges on diagonal steps (below); off in the task kernel, whose
 // register budget it would exceed.
 template <int T, bool BAL>
@@ -793,9 +916,15 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     }
     __syncthreads();
     float *Xo = ltile<T, LT>(X, I, J, nb);
+    // the tile is final: also its FP16 copy, the R0 operand of pairs that have one
+    h16bits *Xh = nullptr;
+    if (!MODE_S && P.Fh >= 0)
+        Xh = (h16bits *)(arena + P.Fh) + utri_ll(i1, j1, P.M) * P.tstride + (Xo - X);
     for (int idx = tid; idx < CH; idx += NT) {
         const int a = idx / Q, c4 = (idx % Q) * 4;
-        *(float4 *)(Xo + a * T + c4) = *(const float4 *)(Xs + a * LX + c4);
+        const float4 v = *(const float4 *)(Xs + a * LX + c4);
+        *(float4 *)(Xo + a * T + c4) = v;
+        if (Xh) *(uint2 *)(Xh + a * T + c4) = make_uint2(pk_pack(v.x, v.y), pk_pack(v.z, v.w));
     }
 }
 
@@ -833,6 +962,18 @@ __global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__
     __shared__ __attribute__((aligned(16))) float smem[OuterLds<T>::FLOATS];
     const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
     outer_body<T>(pairs[it.pair], it, arena, codes, w, smem);
+}
+
+// FP16-operand form of k_outer<64> (every pair of the launch has an FP16 mirror of F).
+// LDS: two FP16 stages, or the 64 x 68 floats of the row-wise atomic combine.
+__global__ __launch_bounds__(kThreads) void k_outer_h(const PairDesc *__restrict__ pairs,
+                                                      const WorkItem *__restrict__ items, float *__restrict__ arena,
+                                                      const int *__restrict__ codes, const float *__restrict__ w)
+{
+    constexpr int BYTES = cmax(2 * HStage::BYTES, 64 * 68 * 4);
+    __shared__ __attribute__((aligned(16))) char smem[BYTES];
+    const WorkItem it = items[xcd_work_index(blockIdx.x, gridDim.x)];
+    outer_body_h<true>(pairs[it.pair], it, arena, codes, w, smem);
 }
 
 // The closure kernels (cross-tile and in-tile) are the critical path of the pipelined
@@ -1178,6 +1319,17 @@ __global__ void k_fill(float *__restrict__ p, long long n, float v)
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// p[0, n) = v and the n2 words after them = the bit pattern v2 (the FP16 mirrors: -inf in
+// both halves), in one launch.
+__global__ void k_fill2(float *__restrict__ p, long long n, float v, long long n2, unsigned v2)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n + n2; i += stride) {
+        if (i < n) p[i] = v;
+        else ((unsigned *)p)[i] = v2;
+    }
+}
+
 // F (tiled) -> packed tri-tri rows: block (i2, outer triangle t).
 __global__ void k_to_packed(const float *__restrict__ arena, PairDesc P, float *__restrict__ out)
 {
@@ -1395,10 +1547,13 @@ __global__ void k_tiled_to_dense(int n, int nb, int T, const float *__restrict__
     }
 
 hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n, float *arena,
-                        const int *codes, const float *w)
+                        const int *codes, const float *w, bool half)
 {
     if (n <= 0) return hipSuccess;
-    BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_outer<TT>, dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
+    if (half && T == 64)
+        hipLaunchKernelGGL(k_outer_h, dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w);
+    else
+        BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_outer<TT>, dim3(n), dim3(kThreads), 0, st, pairs, items, arena, codes, w));
     return hipGetLastError();
 }
 
@@ -1470,6 +1625,14 @@ hipError_t launch_fill(hipStream_t st, float *p, long long n, float v)
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill2(hipStream_t st, float *p, long long n, float v, long long n2, unsigned v2)
+{
+    if (n2 <= 0) return launch_fill(st, p, n, v);
+    long long blocks = (n + n2 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill2, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v, n2, v2);
     return hipGetLastError();
 }
 

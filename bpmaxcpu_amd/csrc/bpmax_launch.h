@@ -5,8 +5,10 @@
 
 namespace bpx {
 
+// half: every pair of the launch has an FP16 mirror of F (PairDesc::Fh); with T = 64 the
+// R0 operands then come from the mirrors (k_outer_h)
 hipError_t launch_outer(hipStream_t st, int T, const PairDesc *pairs, const WorkItem *items, int n,
-                        float *arena, const int *codes, const float *w);
+                        float *arena, const int *codes, const float *w, bool half = false);
 hipError_t launch_inner_ext(hipStream_t st, int T, bool single_strand, const PairDesc *pairs,
                             const WorkItem *items, int n, float *arena);
 // F: wide = 1024-lane workgroups (one cell per lane), else 512 (two cells per lane) or, with
@@ -23,6 +25,8 @@ hipError_t launch_tasks(hipStream_t st, int T, const PairDesc *pairs, const Work
 hipError_t launch_s1(hipStream_t st, const PairDesc *pairs, int npairs, float *arena, const int *codes,
                      const float *w);
 hipError_t launch_fill(hipStream_t st, float *p, long long n, float v);
+// p[0, n) = v, then n2 words of the bit pattern v2
+hipError_t launch_fill2(hipStream_t st, float *p, long long n, float v, long long n2, unsigned v2);
 hipError_t launch_to_packed(hipStream_t st, const float *arena, const PairDesc &P, float *out);
 hipError_t launch_to_dense(hipStream_t st, const float *arena, const PairDesc &P, float *out);
 hipError_t launch_scores(hipStream_t st, const PairDesc *pairs, int npairs, const float *arena, float *scores);

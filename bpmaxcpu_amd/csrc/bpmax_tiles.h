@@ -471,4 +471,126 @@ __device__ __forceinline__ void dl_product(float (&acc)[4][4], const float *stag
     for (int k = lo; k < hi; k += 16) dl_prod16<RS>(acc, stage, r0, c0, k - 32 * kh);
 }
 
+// ------------------------------------------------- FP16 operands (64 x 64 R0, exact scores)
+// When every score of a pair is an integer multiple of 2^-p below 2^(11-p), FP16 holds all
+// of them exactly, every R0 sum (bounded by the cell's final value) is exact, and max-plus
+// runs on packed halves: per two k and two columns, v_pk_add_f16 x 2 (the A value broadcast
+// to both halves by op_sel) + v_pk_maximum3_f16 -- 4 cell updates in 3 VALU instructions
+// against 2 in 3 for FP32, with half the LDS and HBM bytes per operand.  The operands come
+// from the FP16 mirror of F (PairDesc::Fh), same tiled layout, 64 halves per tile row.
+//
+// Stage layout (bytes), one 32-deep half of a K step:
+//   A  4 blocks (block w: rows 16w .. 16w+15, staged by wave w) of 16 rows x 4 chunks of 16 B
+//      (8 consecutive k); block w starts at w*ABLK (1088: each block 64 B further into the
+//      256 B bank window), chunk c of block row rr at slot 4*rr + (c ^ ((rr >> 2) & 3)), so the
+//      eight 4-row groups one wave reads for a chunk fall in eight different bank groups;
+//   B  33 rows x 128 B: rows 32*kh .. 32*kh+32 of the unshifted B operand, row-major;
+//   S1 16 B: the scalar R3/R4 terms that travel with the stage (FP32).
+typedef unsigned short h16bits;
+struct HStage {
+    static constexpr int ABLK = 1088, A = 0, B = 4 * ABLK, S1 = B + 33 * 128, BYTES = S1 + 16;
+};
+static_assert(HStage::BYTES % 16 == 0, "stages stay 16-byte aligned");
+
+// A row of FP16 -inf in device memory (staged rows that lie outside the triangle).
+static __device__ __attribute__((aligned(256))) const unsigned kNegInfRowH[32] = {
+#define BPX_HI4 0xFC00FC00u, 0xFC00FC00u, 0xFC00FC00u, 0xFC00FC00u
+    BPX_HI4, BPX_HI4, BPX_HI4, BPX_HI4, BPX_HI4, BPX_HI4, BPX_HI4, BPX_HI4
+#undef BPX_HI4
+};
+
+// (a.lo + b.lo, a.lo + b.hi) and (a.hi + b.lo, a.hi + b.hi): A's k (resp. k+1) value broadcast
+// against two B columns.
+__device__ __forceinline__ unsigned pk_add_lo(unsigned a, unsigned b)
+{
+    unsigned r;
+    asm("v_pk_add_f16 %0, %1, %2 op_sel_hi:[0,1]" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ unsigned pk_add_hi(unsigned a, unsigned b)
+{
+    unsigned r;
+    asm("v_pk_add_f16 %0, %1, %2 op_sel:[1,0] op_sel_hi:[1,1]" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ unsigned pk_add(unsigned a, unsigned b)
+{
+    unsigned r;
+    asm("v_pk_add_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// acc = max(acc, x, y) on both halves, one v_pk_maximum3_f16 (no NaN ever occurs here)
+__device__ __forceinline__ void pk_max3_into(unsigned &acc, unsigned x, unsigned y)
+{
+    asm("v_pk_maximum3_f16 %0, %0, %1, %2" : "+v"(acc) : "v"(x), "v"(y));
+}
+__device__ __forceinline__ unsigned pk_pack(float lo, float hi)
+{
+    return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(lo, hi));
+}
+__device__ __forceinline__ float pk_lo(unsigned v) { return (float)__builtin_bit_cast(_Float16, (h16bits)(v & 0xFFFFu)); }
+__device__ __forceinline__ float pk_hi(unsigned v) { return (float)__builtin_bit_cast(_Float16, (h16bits)(v >> 16)); }
+
+// Stage the A half [32*kh, 32*kh+32) of the 64 x 64 FP16 tile At: wave w stages rows 16w ..
+// 16w+15, lane l row 16w + l/4 at slot l (its chunk is l%4 XOR the row's swizzle).
+__device__ __forceinline__ void h_stage_A(char *st, const h16bits *At, int kh, int w, int l)
+{
+    const int rr = l >> 2, c = (l & 3) ^ ((rr >> 2) & 3);
+    g2lds16(At + (16 * w + rr) * 64 + 32 * kh + 8 * c, st + HStage::A + w * HStage::ABLK);
+}
+// Stage rows 32*kh + q, q in [8w, 8w+8), of the FP16 tile Bt: lane l row 8w + l/8, chunk l%8.
+__device__ __forceinline__ void h_stage_B32(char *st, const h16bits *Bt, int kh, int w, int l)
+{
+    g2lds16(Bt + (32 * kh + 8 * w + (l >> 3)) * 64 + 8 * (l & 7), st + HStage::B + w * 1024);
+}
+
+// Per-lane A addressing of rows r0 .. r0+3: block row base and swizzle.
+struct HRows {
+    int base[4], sw[4];
+};
+__device__ __forceinline__ HRows h_rows(int r0)
+{
+    HRows h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + i, rr = r & 15;
+        h.base[i] = HStage::A + (r >> 4) * HStage::ABLK + 64 * rr;
+        h.sw[i] = (rr >> 2) & 3;
+    }
+    return h;
+}
+
+// acc[i][p] (columns c0+2p, c0+2p+1 packed) max= A[r0+i][k] + B[k+RS][c] over the 8 k from
+// kq (0, 8, 16 or 24) of the stage's half: four 16 B A reads, eight 8 B B reads, 96 VALU.
+template <int RS>
+__device__ __forceinline__ void h_prod8(unsigned (&acc)[4][2], const char *st, const HRows &rows, int c0, int kq)
+{
+    const int c = kq >> 3;
+    uint4 a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *(const uint4 *)(st + rows.base[i] + 16 * (c ^ rows.sw[i]));
+    uint2 b[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = *(const uint2 *)(st + HStage::B + (kq + RS + j) * 128 + 2 * c0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const unsigned av = m == 0 ? a[i].x : m == 1 ? a[i].y : m == 2 ? a[i].z : a[i].w;
+            pk_max3_into(acc[i][0], pk_add_lo(av, b[2 * m].x), pk_add_hi(av, b[2 * m + 1].x));
+            pk_max3_into(acc[i][1], pk_add_lo(av, b[2 * m].y), pk_add_hi(av, b[2 * m + 1].y));
+        }
+    }
+}
+
+// h_prod8 over [lo, hi) (multiples of 8) clipped to the stage's half [32*kh, 32*kh+32).
+template <int RS>
+__device__ __forceinline__ void h_product(unsigned (&acc)[4][2], const char *st, const HRows &rows, int c0, int kh,
+                                          int lo, int hi)
+{
+    lo = lo > 32 * kh ? lo : 32 * kh;
+    hi = hi < 32 * kh + 32 ? hi : 32 * kh + 32;
+    for (int k = lo; k < hi; k += 8) h_prod8<RS>(acc, st, rows, c0, k - 32 * kh);
+}
+
 } // namespace bpx

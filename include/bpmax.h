@@ -311,6 +311,18 @@ int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *t);
 /* Enable or disable (default) HIP-graph capture and replay for BPMAX_SCHED_PIPE; without
  * timing only.  BPMAX_GRAPHS=1 in the environment sets the default. */
 int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);
+/* Enable (default) or disable FP16 operands for R0.  When every pair of a solve (or batch)
+ * has 64 x 64 storage tiles and a score bound that FP16 represents exactly -- the weights
+ * are multiples of 2^-p and wmax * ((M+N)/2 + 1) * 2^p <= 2^11 -- the inner closure also
+ * writes each finished F tile as FP16 into a mirror (half the bytes of F, extra device
+ * memory) and the R0 launches multiply those on packed halves (v_pk_add_f16 +
+ * v_pk_maximum3_f16: 4 cell updates per 3 VALU instructions).  Every value is then exact,
+ * so F and the score are bit-identical either way; other solves, the serving loop and the
+ * no-allocation solve use FP32 operands.  BPMAX_R0_HALF=0 in the environment sets the
+ * default.  No counterpart in the reference (its R0 is FP32 AVX2). */
+int bpmax_ctx_set_r0_half(bpmax_ctx *ctx, int enable);
+/* 1 if the last solve of the context ran its R0 launches on FP16 operands, else 0. */
+int bpmax_ctx_r0_half_used(const bpmax_ctx *ctx);
 /* The hipStream_t the context launches on (as void*). */
 void *bpmax_ctx_stream(bpmax_ctx *ctx);
 int bpmax_ctx_synchronize(bpmax_ctx *ctx);

@@ -31,8 +31,8 @@ def eng():
 
 
 @pytest.fixture(scope="module",
-                params=["pipe", "pipe-eager", "pipe-ahead", "pipe-push", "pipe-t32", "pipe-r0whole", "launch-narrow",
-                        "launch-wide", "launch-ahead", "tasks"])
+                params=["pipe", "pipe-f32", "pipe-eager", "pipe-ahead", "pipe-push", "pipe-t32", "pipe-r0whole",
+                        "launch-narrow", "launch-wide", "launch-ahead", "tasks"])
 def eng2(request):
     """An engine under each schedule:
     * pipe: the two-stream pipeline with the default settings (inner closure in one launch
@@ -48,7 +48,9 @@ def eng2(request):
       tile candidate);
     * pipe-r0whole: the pipeline with each outer diagonal's R0 run whole after the closure
       of the diagonal below (instead of its bulk one diagonal ahead), in 2 ranges;
-    * tasks: the dependency-tile scheduler."""
+    * pipe-f32: the default pipeline with FP32 R0 operands for every pair (the others use the
+      FP16 mirrors wherever the score bound allows, which is every fixture here);
+    * tasks: the dependency-tile scheduler (FP32 R0 operands)."""
     push = request.param.endswith("push")
     env = {"BPMAX_INTRA_BLOCKED": "0"} if push else ({"BPMAX_R0_AHEAD": "0"} if request.param == "pipe-r0whole" else {})
     os.environ.update(env)  # read when the context is created
@@ -71,7 +73,8 @@ def eng2(request):
         e.set_default_tuning({"schedule": "pipe", "outer_split": 2, "outer_items": 2048, "graphs": False,
                               "intra_wide": 0 if push else 256, "outer_kgroup": 1, "closure_ahead": ahead})
     e.set_schedule(sched)
-    e.set_graphs(request.param == "pipe")
+    e.set_graphs(request.param in ("pipe", "pipe-f32"))
+    e.set_r0_half(request.param != "pipe-f32")
     yield e
     e.close()
 
@@ -191,6 +194,36 @@ def test_invalid_inputs_are_rejected(eng):
     # the context is still usable afterwards
     _, s = eng.solve(np.array([0], np.int32), np.array([3], np.int32))
     assert s == 1.0
+
+
+def test_r0_operand_precision_follows_the_score_bound(eng):
+    """R0 runs on FP16 operands exactly when every score is an FP16 integer multiple of 2^-p
+    (wmax * ((M+N)/2 + 1) * 2^p <= 2048) and the pair has 64 x 64 tiles; F and the score are
+    bit-identical to the oracle either way."""
+    rng = np.random.default_rng(5)
+    a, b = rng.integers(0, 4, 16).astype(np.int32), rng.integers(0, 4, 100).astype(np.int32)
+    intra = B.reference_table()  # GC 3, AU 1, GU 1.5: p = 1
+    inter = intra
+    # bounds at 16 x 100 ((16+100)/2 + 1 = 59 pairs): 3 * 59 * 2^1 = 354; x16: 48 * 59 = 2832
+    # (GU 24, p = 0); x0.25: 0.75 * 59 * 2^3 = 354 (GU 0.375, p = 3)
+    cases = [(intra, inter, True), (intra * 16, inter * 16, False), (intra * 0.25, inter * 0.25, True)]
+    for wi, we, expect in cases:
+        _say(f"precision 16x100 wmax {float(max(wi.max(), we.max()))}")
+        F, sc = eng.solve(a, b, wi, we)
+        assert eng.r0_half_used() == expect
+        Fo, so = O.bpmax(a, b, wi, we)
+        assert sc == so and _bits_equal(F, Fo)
+    # N <= 64 pairs use 32 x 32 tiles: FP32 operands
+    eng.solve(a, b[:60], intra, inter)
+    assert not eng.r0_half_used()
+    eng.set_r0_half(False)
+    try:
+        F, sc = eng.solve(a, b, intra, inter)
+        assert not eng.r0_half_used()
+        Fo, so = O.bpmax(a, b, intra, inter)
+        assert sc == so and _bits_equal(F, Fo)
+    finally:
+        eng.set_r0_half(True)
 
 
 @pytest.mark.parametrize("n,batch", [(1, 1), (5, 3), (64, 2), (65, 2), (130, 3), (512, 2)])
