@@ -1133,14 +1133,18 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
     for (const Launch &L : S.launches)
         if (L.d1 >= 0 && on_r0_stream(L) && L.d1 - lag(L) >= 0) waited[L.group][in_need(L)] = 1;
 
-    if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0,
-                    [&] { return launch_fill2(sO, arena, fl, -INFINITY, ctx->fill_half, 0xFC00FC00u); })) != BPMAX_OK)
+    if ((rc = timed(ctx, sO, BPMAX_K_FILL, 0, [&] { return launch_fill(sO, arena, fl, -INFINITY); })) != BPMAX_OK)
         return rc;
     HIPCHK(ctx, hipEventRecord(ev_fill, sO));
     HIPCHK(ctx, hipStreamWaitEvent(sI, ev_fill, 0));
-    // S1 is read by the R0 launches only: built on their stream, beside the S2 chain
+    // S1 is read by the R0 launches only: built on their stream, beside the S2 chain; so is
+    // the FP16 mirrors' -inf fill, which the first closure (after the first R0 launch) needs
     if ((rc = timed(ctx, sO, BPMAX_K_S1, 0, [&] { return launch_s1(sO, dP, npairs, arena, dcodes, dw); })) !=
         BPMAX_OK)
+        return rc;
+    if (ctx->fill_half > 0 &&
+        (rc = timed(ctx, sO, BPMAX_K_FILL, 0,
+                    [&] { return launch_fill2(sO, arena + fl, 0, 0.f, ctx->fill_half, 0xFC00FC00u); })) != BPMAX_OK)
         return rc;
     for (const Launch &L : S.launches)
         if (L.d1 < 0 && (rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
