@@ -254,10 +254,11 @@ __device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__re
 }
 
 // FP16 form of outer_body_dl (pairs whose scores FP16 holds exactly, PairDesc::Fh >= 0):
-// the R0 operands are staged from the FP16 mirror of F (HStage: 8.6 KB per half instead of
-// 16.5) and multiplied on packed halves (h_product: 4 cell updates per 3 VALU instructions);
-// R3/R4 read the same staged tiles.  The lane's 4 x 4 cells are held as 4 x 2 packed pairs
-// and turned back into FP32 (exactly) for the epilogue and the store, which are
+// the R0 operands are staged from the FP16 mirror of F, a whole 64-deep K step per stage
+// (HStage, 16.8 KB: the LDS of one 32-deep FP32 half), so each step takes one barrier and one
+// load wait; they are multiplied on packed halves (h_product: 4 cell updates per 3 VALU
+// instructions).  R3/R4 read the same staged tiles.  The lane's 4 x 4 cells are held as 4 x 2
+// packed pairs and turned back into FP32 (exactly) for the epilogue and the store, which are
 // outer_body_dl's.
 template <bool BAL>
 __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__restrict__ arena,
@@ -280,46 +281,41 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
     const OuterRanges g = outer_ranges<T, BAL>(it, nK, Kb, wv);
     int hr0 = 0, hc0 = 0;
     if (g.hw >= 0) lane_block<T>(hr0, hc0, g.hw * 64 + l);
-    const HRows rows = h_rows(r0), hrows = h_rows(hr0);
+    const HRows rows = h_rows(r0);
     // padding (N not a multiple of T): see outer_body_rs
     const int kreal = (P.N - (nb - 1) * T + H - 1) / H * H;
     auto quad_pad = [&](int x) {
         return (I == nb - 1 && I * T + (x >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (x & 1) * QS >= P.N);
     };
     const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
-    auto issue = [&](int h) {
-        const int s = h >> 1, kh = h & 1, k1 = outer_k1(it, s, nK), K = Kb + s % nK;
-        char *st = smem + (h & 1) * HStage::BYTES;
+    auto issue = [&](int s) {
+        const int k1 = outer_k1(it, s, nK), K = Kb + s % nK;
+        char *st = smem + (s & 1) * HStage::BYTES;
         const h16bits *Bf = tri_h(k1 + 1, j1);
-        h_stage_A(st, tri_h(i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), kh, wv, l);
-        h_stage_B32(st, Bf + utri_ll(K, J, nb) * (long long)(T * T), kh, wv, l);
-        if (wv == 0 && l < 8) { // B row 32 of the half: row 32 of tile K, or row 0 of tile K+1 (-inf past J)
-            const h16bits *src = kh == 0 ? Bf + utri_ll(K, J, nb) * (long long)(T * T) + 32 * T
-                                         : (K + 1 <= J ? Bf + utri_ll(K + 1, J, nb) * (long long)(T * T)
-                                                       : (const h16bits *)kNegInfRowH);
-            g2lds16(src + 8 * l, st + HStage::B + 32 * 128);
+        h_stage_A(st, tri_h(i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), wv, l);
+        h_stage_B(st, Bf + utri_ll(K, J, nb) * (long long)(T * T), wv, l);
+        if (wv == 0 && l < 8) { // B row 64: row 0 of tile K+1 (-inf past J)
+            const h16bits *src =
+                K + 1 <= J ? Bf + utri_ll(K + 1, J, nb) * (long long)(T * T) : (const h16bits *)kNegInfRowH;
+            g2lds16(src + 8 * l, st + HStage::B + 64 * 128);
         }
-        if (kh == 0 && l == 0) {
+        if (l == 0) {
             if (wv == 1 && K == I) g2lds4(S1 + i1 * M + k1, st + HStage::S1);
             if (wv == 2 && K == J) g2lds4(S1 + (k1 + 1) * M + j1, st + HStage::S1 + 4);
         }
     };
-    const int h0 = 2 * it.step_begin, h1 = 2 * it.step_end;
-    if (h0 < h1) issue(h0);
-    unsigned s1a = 0xFC00FC00u, s1b = 0xFC00FC00u;
-    for (int h = h0; h < h1; ++h) {
-        const int s = h >> 1, kh = h & 1, K = Kb + s % nK;
-        const char *st = smem + (h & 1) * HStage::BYTES;
+    if (it.step_begin < it.step_end) issue(it.step_begin);
+    for (int s = it.step_begin; s < it.step_end; ++s) {
+        const int K = Kb + s % nK;
+        const char *st = smem + (s & 1) * HStage::BYTES;
         vm_wait_all();
         __syncthreads();
-        if (kh == 0) { // read before the next loads are issued into the other stage
-            const float a = *(const float *)(st + HStage::S1), b = *(const float *)(st + HStage::S1 + 4);
-            s1a = pk_pack(a, a);
-            s1b = pk_pack(b, b);
-        }
-        if (h + 1 < h1) issue(h + 1); // in flight while this half computes
-        if (K == J && kh == (wv & 1)) { // R4: A(r, c) = F(i1,k1)[I*T + r][J*T + c], columns c0 .. c0+3
-            const int c = (c0 - 32 * kh) >> 3, off = 2 * (c0 & 7);
+        // read before the next loads are issued into the other stage
+        const float sa = *(const float *)(st + HStage::S1), sb = *(const float *)(st + HStage::S1 + 4);
+        if (s + 1 < it.step_end) issue(s + 1); // in flight while this step computes
+        if (K == J) { // R4: A(r, c) = F(i1,k1)[I*T + r][J*T + c], columns c0 .. c0+3
+            const unsigned s1b = pk_pack(sb, sb);
+            const int c = c0 >> 3, off = 2 * (c0 & 7);
 #pragma unroll
             for (int i = 0; i < R; ++i) {
                 const uint2 v = *(const uint2 *)(st + rows.base[i] + 16 * (c ^ rows.sw[i]) + off);
@@ -328,10 +324,11 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
                 pk_max3_into(acc[i][1], t1, t1);
             }
         }
-        if (K == I && kh == (wv >> 1)) { // R3: B row (r - 32kh) = F(k1+1,j1)[I*T + r][J*T + c]
+        if (K == I) { // R3: B row r = F(k1+1,j1)[I*T + r][J*T + c]
+            const unsigned s1a = pk_pack(sa, sa);
 #pragma unroll
             for (int i = 0; i < R; ++i) {
-                const uint2 v = *(const uint2 *)(st + HStage::B + (r0 + i - 32 * kh) * 128 + 2 * c0);
+                const uint2 v = *(const uint2 *)(st + HStage::B + (r0 + i) * 128 + 2 * c0);
                 const unsigned t0 = pk_add(s1a, v.x), t1 = pk_add(s1a, v.y);
                 pk_max3_into(acc[i][0], t0, t0);
                 pk_max3_into(acc[i][1], t1, t1);
@@ -340,8 +337,8 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
         int kb = g.mb, ke = g.me;
         if (!g.balance) quad_k_range<T>(K == I, K == J, false, kb, ke);
         const int klim = K == nb - 1 ? kreal : T;
-        if (!own_pad) h_product<1>(acc, st, rows, c0, kh, kb, ke < klim ? ke : klim);
-        if (g.hw >= 0 && !help_pad) h_product<1>(acc2, st, hrows, hc0, kh, g.hb, g.he < klim ? g.he : klim);
+        if (!own_pad) h_product<1>(acc, st, rows, c0, kb, ke < klim ? ke : klim);
+        if (g.hw >= 0 && !help_pad) h_product<1>(acc2, st, h_rows(hr0), hc0, g.hb, g.he < klim ? g.he : klim);
     }
     if (g.balance) { // helpers hand their slices to the helped waves (as outer_merge, packed)
         unsigned *x = (unsigned *)smem;
@@ -965,7 +962,8 @@ __global__ __launch_bounds__(kThreads) void k_outer(const PairDesc *__restrict__
 }
 
 // FP16-operand form of k_outer<64> (every pair of the launch has an FP16 mirror of F).
-// LDS: two FP16 stages, or the 64 x 68 floats of the row-wise atomic combine.
+// LDS: two 64-deep FP16 stages (also room for the 64 x 68 floats of the row-wise atomic
+// combine).
 __global__ __launch_bounds__(kThreads) void k_outer_h(const PairDesc *__restrict__ pairs,
                                                       const WorkItem *__restrict__ items, float *__restrict__ arena,
                                                       const int *__restrict__ codes, const float *__restrict__ w)

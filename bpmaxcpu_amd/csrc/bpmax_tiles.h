@@ -479,16 +479,18 @@ __device__ __forceinline__ void dl_product(float (&acc)[4][4], const float *stag
 // against 2 in 3 for FP32, with half the LDS and HBM bytes per operand.  The operands come
 // from the FP16 mirror of F (PairDesc::Fh), same tiled layout, 64 halves per tile row.
 //
-// Stage layout (bytes), one 32-deep half of a K step:
-//   A  4 blocks (block w: rows 16w .. 16w+15, staged by wave w) of 16 rows x 4 chunks of 16 B
-//      (8 consecutive k); block w starts at w*ABLK (1088: each block 64 B further into the
-//      256 B bank window), chunk c of block row rr at slot 4*rr + (c ^ ((rr >> 2) & 3)), so the
+// Stage layout (bytes), one whole 64-deep K step (FP16 halves the bytes of a step, so two
+// such stages take the LDS of two 32-deep FP32 ones: one barrier and one load wait per step):
+//   A  4 blocks (block w: rows 16w .. 16w+15, staged by wave w) of 16 rows x 8 chunks of 16 B
+//      (8 consecutive k); block w starts at w*ABLK (2112: each block 64 B further into the
+//      256 B bank window), chunk c of block row rr at slot 8*rr + (c ^ ((rr >> 2) & 3)), so the
 //      eight 4-row groups one wave reads for a chunk fall in eight different bank groups;
-//   B  33 rows x 128 B: rows 32*kh .. 32*kh+32 of the unshifted B operand, row-major;
+//   B  65 rows x 128 B: rows 0 .. 63 of the unshifted B tile and, as row 64, row 0 of the
+//      next tile down (R0's k2+1 shift), row-major;
 //   S1 16 B: the scalar R3/R4 terms that travel with the stage (FP32).
 typedef unsigned short h16bits;
 struct HStage {
-    static constexpr int ABLK = 1088, A = 0, B = 4 * ABLK, S1 = B + 33 * 128, BYTES = S1 + 16;
+    static constexpr int ABLK = 2112, A = 0, B = 4 * ABLK, S1 = B + 65 * 128, BYTES = S1 + 16;
 };
 static_assert(HStage::BYTES % 16 == 0, "stages stay 16-byte aligned");
 
@@ -531,17 +533,24 @@ __device__ __forceinline__ unsigned pk_pack(float lo, float hi)
 __device__ __forceinline__ float pk_lo(unsigned v) { return (float)__builtin_bit_cast(_Float16, (h16bits)(v & 0xFFFFu)); }
 __device__ __forceinline__ float pk_hi(unsigned v) { return (float)__builtin_bit_cast(_Float16, (h16bits)(v >> 16)); }
 
-// Stage the A half [32*kh, 32*kh+32) of the 64 x 64 FP16 tile At: wave w stages rows 16w ..
-// 16w+15, lane l row 16w + l/4 at slot l (its chunk is l%4 XOR the row's swizzle).
-__device__ __forceinline__ void h_stage_A(char *st, const h16bits *At, int kh, int w, int l)
+// Stage the 64 x 64 FP16 tile At as the A operand: wave w stages rows 16w .. 16w+15, two
+// loads of 8 rows each; lane l of load q takes row 16w + 8q + l/8 at slot 64q + l (its chunk is
+// l%8 XOR the row's swizzle).
+__device__ __forceinline__ void h_stage_A(char *st, const h16bits *At, int w, int l)
 {
-    const int rr = l >> 2, c = (l & 3) ^ ((rr >> 2) & 3);
-    g2lds16(At + (16 * w + rr) * 64 + 32 * kh + 8 * c, st + HStage::A + w * HStage::ABLK);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int rr = 8 * q + (l >> 3), c = (l & 7) ^ ((rr >> 2) & 3);
+        g2lds16(At + (16 * w + rr) * 64 + 8 * c, st + HStage::A + w * HStage::ABLK + 1024 * q);
+    }
 }
-// Stage rows 32*kh + q, q in [8w, 8w+8), of the FP16 tile Bt: lane l row 8w + l/8, chunk l%8.
-__device__ __forceinline__ void h_stage_B32(char *st, const h16bits *Bt, int kh, int w, int l)
+// Stage rows 16w .. 16w+15 of the FP16 tile Bt (B rows 0 .. 63): lane l of load q row
+// 16w + 8q + l/8, chunk l%8.
+__device__ __forceinline__ void h_stage_B(char *st, const h16bits *Bt, int w, int l)
 {
-    g2lds16(Bt + (32 * kh + 8 * w + (l >> 3)) * 64 + 8 * (l & 7), st + HStage::B + w * 1024);
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+        g2lds16(Bt + (16 * w + 8 * q + (l >> 3)) * 64 + 8 * (l & 7), st + HStage::B + (2 * w + q) * 1024);
 }
 
 // Per-lane A addressing of rows r0 .. r0+3: block row base and swizzle.
@@ -554,14 +563,14 @@ __device__ __forceinline__ HRows h_rows(int r0)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int r = r0 + i, rr = r & 15;
-        h.base[i] = HStage::A + (r >> 4) * HStage::ABLK + 64 * rr;
+        h.base[i] = HStage::A + (r >> 4) * HStage::ABLK + 128 * rr;
         h.sw[i] = (rr >> 2) & 3;
     }
     return h;
 }
 
 // acc[i][p] (columns c0+2p, c0+2p+1 packed) max= A[r0+i][k] + B[k+RS][c] over the 8 k from
-// kq (0, 8, 16 or 24) of the stage's half: four 16 B A reads, eight 8 B B reads, 96 VALU.
+// kq (a multiple of 8): four 16 B A reads, eight 8 B B reads, 96 VALU for 128 cell updates.
 template <int RS>
 __device__ __forceinline__ void h_prod8(unsigned (&acc)[4][2], const char *st, const HRows &rows, int c0, int kq)
 {
@@ -583,14 +592,12 @@ __device__ __forceinline__ void h_prod8(unsigned (&acc)[4][2], const char *st, c
     }
 }
 
-// h_prod8 over [lo, hi) (multiples of 8) clipped to the stage's half [32*kh, 32*kh+32).
+// h_prod8 over [lo, hi) (multiples of 8).
 template <int RS>
-__device__ __forceinline__ void h_product(unsigned (&acc)[4][2], const char *st, const HRows &rows, int c0, int kh,
-                                          int lo, int hi)
+__device__ __forceinline__ void h_product(unsigned (&acc)[4][2], const char *st, const HRows &rows, int c0, int lo,
+                                          int hi)
 {
-    lo = lo > 32 * kh ? lo : 32 * kh;
-    hi = hi < 32 * kh + 32 ? hi : 32 * kh + 32;
-    for (int k = lo; k < hi; k += 8) h_prod8<RS>(acc, st, rows, c0, k - 32 * kh);
+    for (int k = lo; k < hi; k += 8) h_prod8<RS>(acc, st, rows, c0, k);
 }
 
 } // namespace bpx
