@@ -7,7 +7,8 @@ inner R1/R2/P2 phase) with the sequences on the host and the F table left reside
 cells = C(M+1,3)C(N+1,3) + 2C(M+1,2)C(N+1,3) + 2C(M+1,3)C(N+1,2).
 
 The JSON line also carries
-  roofline     -- the dominant kernel, k_outer<64> (R0 + init/P1/R3/R4), timed with HIP
+  roofline     -- the dominant kernel, k_outer_h (R0 + init/P1/R3/R4 on FP16 operands, exact
+                  here; k_outer<64> where FP16 is not exact), timed with HIP
                   events on the stream it is launched on during the timed steps:
                   algorithmic flops (2 per cell update: one add, one max) per launch over
                   its average launch duration, against the MI355X FP32 vector peak;
@@ -36,6 +37,7 @@ sys.path.insert(0, REPO)
 TUNING_TABLE = os.path.join(REPO, "bpmaxcpu_amd", "tuning_mi355x.json")
 FP32_VALU_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector peak: 256 CUs x 2.4 GHz x 128 lanes x 2 (FMA)
 VALU_ISSUE_PEAK = 2 * 256 * 2.4e9  # wave64 FP32 VALU instructions per second (2 per clock per CU)
+VALU_ISSUE_PEAK_PK = 256 * 2.4e9   # wave64 packed FP16 VALU instructions per second (half the FP32 rate)
 
 
 def log(msg):
@@ -160,6 +162,7 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None, retune=False):
         _, score = eng.solve(s1, s2, layout=B.LAYOUT_NONE)
     eng.synchronize()
     wall = time.perf_counter() - t0
+    r0_half = eng.r0_half_used()  # R0 of these solves on FP16 operands (exact for this table and shape)
     st = eng.kernel_stats()
     eng.set_timing(False)
     timed_solves = 1
@@ -187,7 +190,8 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None, retune=False):
     avg_s = o["ms"] / max(1, o["launches"]) / 1e3
     achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "k_outer_64x512_traffic.json")
+    kname = "k_outer_h" if r0_half else "k_outer"
+    tpath = os.path.join(REPO, "profiles", f"{kname}_64x512_traffic.json")
     if (M, N) == (64, 512) and os.path.exists(tpath) and o["launches"]:
         # rocprofv3 FETCH_SIZE + WRITE_SIZE of one solve's k_outer launches (copy-calibrated),
         # spread over this run's launches per solve (the tuned R0 split changes their number)
@@ -195,22 +199,25 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None, retune=False):
             traffic = int(round(json.load(fh)["bytes_per_solve"] * timed_solves / o["launches"]))
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_outer<64>: R0 double max-plus + init/P1/R3/R4 (2 flops per cell update)",
+                "kernel": ("k_outer_h" if r0_half else "k_outer<64>") +
+                          ": R0 double max-plus + init/P1/R3/R4 (2 flops per cell update)",
                 "launches": o["launches"], "avg_launch_ms": round(avg_s * 1e3, 4),
                 "flops_per_launch": flops_per_launch}
-    vpath = os.path.join(REPO, "profiles", "k_outer_64x512_valu.json")
+    vpath = os.path.join(REPO, "profiles", f"{kname}_64x512_valu.json")
     if (M, N) == (64, 512) and os.path.exists(vpath) and o["launches"] and avg_s > 0:
         # rocprofv3 SQ_INSTS_VALU of one solve (calibrated to count wave instructions exactly)
-        # spread over this run's launches per solve, against the issue peak of two wave64
-        # FP32 instructions per clock per CU
+        # spread over this run's launches per solve, against the issue peak of the kernel's
+        # instruction kind: two wave64 FP32 instructions per clock per CU, or one packed FP16
+        # instruction (half rate; bench.py --calibrate-valu measures both)
         with open(vpath) as fh:
             per_solve = json.load(fh)["sq_insts_valu_per_solve"]
         per_launch = per_solve * timed_solves / o["launches"]
         rate = per_launch / avg_s
+        peak = VALU_ISSUE_PEAK_PK if r0_half else VALU_ISSUE_PEAK
         roofline["valu_issue"] = {"wave_instr_per_launch": per_launch, "achieved": rate,
-                                  "peak": VALU_ISSUE_PEAK, "unit": "wave VALU instructions/s",
-                                  "frac": round(rate / VALU_ISSUE_PEAK, 4),
-                                  "source": "profiles/k_outer_64x512_valu.md"}
+                                  "peak": peak, "unit": "wave VALU instructions/s",
+                                  "frac": round(rate / peak, 4),
+                                  "source": f"profiles/{kname}_64x512_valu.md"}
 
     # one-stream schedule, for the phase breakdown without overlap
     eng.set_tuning(M, N, 1, {"schedule": "launch", "outer_split": 1, "outer_items": tuned["outer_items"],
@@ -246,6 +253,7 @@ def pair_leg(B, eng, M, N, steps, warmup, schedule=None, retune=False):
                      "per-tile counters (agent-scope acquire/release per task)"}
     log(f"dependency-tile scheduler: {ms_tasks:.1f} ms ({ts['tasks']} tasks, {ts['wait_polls']:.3g} waiting polls)")
     return {"score": score, "ms_per_step": ms_per_step, "ms_events": ms_events, "gcups": gcups,
+            "r0_half": r0_half,
             "roofline": roofline, "phases_pipelined": phases, "phases_one_stream": serial, "tuning": tuned,
             "dependency_tiles": tasks}
 
@@ -488,23 +496,27 @@ def main():
                     help="only run a plain copy kernel over MIB MiB (reads MIB, writes MIB): the "
                          "reference point that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     ap.add_argument("--calibrate-valu", action="store_true",
-                    help="only run the VALU counter-calibration kernel (libbpmax_probe.so): two launches "
-                         "of 2048 workgroups x 4 waves with 32 x 1000 and 32 x 4000 VALU instructions per "
-                         "wave, the reference points that scale rocprofv3 SQ_INSTS_VALU to wave instructions")
+                    help="only run the VALU counter-calibration kernels (libbpmax_probe.so): FP32, then packed "
+                         "FP16, two launches each of 2048 workgroups x 4 waves with 32 x 1000 and 32 x 4000 VALU "
+                         "instructions per wave, the reference points that scale rocprofv3 SQ_INSTS_VALU to wave "
+                         "instructions and measure each kind's issue rate")
     args = ap.parse_args()
     if args.calibrate_valu:
         import ctypes
         P = ctypes.CDLL(os.path.join(REPO, "bpmaxcpu_amd", "libbpmax_probe.so"))
-        P.bpmax_probe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         out = []
-        for iters in (1000, 4000):
-            ms = ctypes.c_double()
-            rc = P.bpmax_probe_valu(2048, iters, ctypes.byref(ms))
-            if rc != 0:
-                raise SystemExit(f"probe failed: {rc}")
-            waves = 2048 * 4
-            out.append({"iters": iters, "waves": waves, "valu_per_wave_loop": 32 * iters, "ms": round(ms.value, 4),
-                        "wave_valu_per_clock_per_cu": round(waves * 32 * iters / (ms.value * 1e-3) / 2.4e9 / 256, 3)})
+        for name in ("bpmax_probe_valu", "bpmax_probe_valu_pk"):
+            fn = getattr(P, name)
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+            for iters in (1000, 4000):
+                ms = ctypes.c_double()
+                rc = fn(2048, iters, ctypes.byref(ms))
+                if rc != 0:
+                    raise SystemExit(f"probe failed: {rc}")
+                waves = 2048 * 4
+                out.append({"probe": "packed fp16" if name.endswith("_pk") else "fp32", "iters": iters,
+                            "waves": waves, "valu_per_wave_loop": 32 * iters, "ms": round(ms.value, 4),
+                            "wave_valu_per_clock_per_cu": round(waves * 32 * iters / (ms.value * 1e-3) / 2.4e9 / 256, 3)})
         print(json.dumps({"valu_probe": out}), flush=True)
         return
     if args.calibrate_copy:
@@ -556,7 +568,11 @@ def main():
 
     line = {"metric": "BPMax GCUPS at M=64,N=512", "value": round(res["gcups"], 3), "unit": "GCUPS", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["ms_per_step"], 3),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f16/f32" if res["r0_half"] else "f32",
+            "dtype_note": ("R0 (k_outer_h) multiplies packed FP16 operands, exact here: scores are multiples of "
+                           "2^-1 and at most 3 * 289 < 2^10; the rest is FP32; F and the score are "
+                           "bit-identical to the reference's FP32") if res["r0_half"] else "FP32 throughout",
             "data": "synthetic: uniform i.i.d. ACGU from seed 7, reference weight table (GC=3, AU=1, GU=1.5)",
             "config": {"workload": f"plain BPMax, one pair M={M} N={N}, score-only output, F resident in HBM",
                        "M": M, "N": N, "pairs_per_step": 1, "score": res["score"],
