@@ -253,6 +253,49 @@ __device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__re
     else store_acc<T, T>(dst, acc, r0, c0, false);
 }
 
+// Cell mapping of an FP16 R0 item.  With one K per item every step of the item is of one
+// kind.  K = I only (A a diagonal tile: row r needs k >= r): wave w owns rows 16w .. 16w+15
+// (lane l: rows 16w + 4(l/16), columns 4(l%16)) and needs k in [16w, 64).  K = J only (B the
+// shifted diagonal tile: column c needs k < c): columns 16w .. 16w+15 (lane l: rows 4(l/4),
+// columns 16w + 4(l%4)) and k in [0, 16w+16).  Those ranges, 64, 48, 32 and 16 k, are balanced
+// to 40 per wave: the two longest hand 8-k-aligned slices to the two shortest, which run them on
+// the longer waves' cells into their second accumulator.  Other items keep the quadrants of
+// outer_ranges (48 k per wave at most on a K = I or K = J step).
+enum : int { HMAP_QUAD, HMAP_ROWS, HMAP_COLS };
+__device__ __forceinline__ void h_cells(int map, int w, int l, int &r0, int &c0)
+{
+    if (map == HMAP_ROWS) {
+        r0 = 16 * w + 4 * (l >> 4);
+        c0 = 4 * (l & 15);
+    } else if (map == HMAP_COLS) {
+        r0 = 4 * (l >> 2);
+        c0 = 16 * w + 4 * (l & 3);
+    } else {
+        lane_block<64>(r0, c0, w * 64 + l);
+    }
+}
+__device__ __forceinline__ OuterRanges h_band_ranges(int map, int w)
+{
+    OuterRanges g;
+    g.balance = true;
+    if (map == HMAP_ROWS) { // needs: w0 [0,64) w1 [16,64) w2 [32,64) w3 [48,64)
+        constexpr int mb[4] = {0, 16, 32, 48}, me[4] = {40, 56, 64, 64};
+        g.mb = mb[w];
+        g.me = me[w];
+        if (w == 2) g.hw = 1, g.hb = 56, g.he = 64;
+        if (w == 3) g.hw = 0, g.hb = 40, g.he = 64;
+        g.helped = w < 2;
+    } else { // needs: w0 [0,16) w1 [0,32) w2 [0,48) w3 [0,64)
+        constexpr int me[4] = {16, 32, 40, 40};
+        g.mb = 0;
+        g.me = me[w];
+        if (w == 0) g.hw = 3, g.hb = 40, g.he = 64;
+        if (w == 1) g.hw = 2, g.hb = 40, g.he = 48;
+        g.helped = w >= 2;
+    }
+    return g;
+}
+
 // FP16 form of outer_body_dl (pairs whose scores FP16 holds exactly, PairDesc::Fh >= 0):
 // the R0 operands are staged from the FP16 mirror of F, a whole 64-deep K step per stage
 // (HStage, 16.8 KB: the LDS of one 32-deep FP32 half), so each step takes one barrier and one
@@ -265,8 +308,6 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
                              const int *__restrict__ codes, const float *__restrict__ w, char *smem)
 {
     constexpr int T = 64, R = 4, QS = Tc<T>::QS, H = QS / 2;
-    int r0, c0;
-    lane_block<T>(r0, c0);
     const int i1 = it.i1, j1 = it.j1, I = it.I, J = it.J, nb = P.nb, M = P.M;
     const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
     const float *S1 = arena + P.S1;
@@ -278,13 +319,19 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
     unsigned acc[R][2], acc2[R][2];
 #pragma unroll
     for (int i = 0; i < R; ++i) acc[i][0] = acc[i][1] = acc2[i][0] = acc2[i][1] = 0xFC00FC00u;
-    const OuterRanges g = outer_ranges<T, BAL>(it, nK, Kb, wv);
-    int hr0 = 0, hc0 = 0;
-    if (g.hw >= 0) lane_block<T>(hr0, hc0, g.hw * 64 + l);
+    const bool one_k = BAL && nK == 1 && it.step_begin < it.step_end;
+    const int map = !one_k || Kb == I && Kb == J ? HMAP_QUAD : (Kb == I ? HMAP_ROWS : (Kb == J ? HMAP_COLS : HMAP_QUAD));
+    const OuterRanges g = map == HMAP_QUAD ? outer_ranges<T, BAL>(it, nK, Kb, wv) : h_band_ranges(map, wv);
+    int r0, c0, hr0 = 0, hc0 = 0;
+    h_cells(map, wv, l, r0, c0);
+    if (g.hw >= 0) h_cells(map, g.hw, l, hr0, hc0);
     const HRows rows = h_rows(r0);
-    // padding (N not a multiple of T): see outer_body_rs
+    // padding (N not a multiple of T): see outer_body_rs; a wave whose quadrant or band lies
+    // wholly in padding rows or columns runs no product
     const int kreal = (P.N - (nb - 1) * T + H - 1) / H * H;
     auto quad_pad = [&](int x) {
+        if (map == HMAP_ROWS) return I == nb - 1 && I * T + 16 * x >= P.N;
+        if (map == HMAP_COLS) return J == nb - 1 && J * T + 16 * x >= P.N;
         return (I == nb - 1 && I * T + (x >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (x & 1) * QS >= P.N);
     };
     const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
