@@ -1148,6 +1148,10 @@ static int enqueue_pipe(bpmax_ctx *ctx, int npairs, const Schedule &S, const Pai
         return rc;
     for (const Launch &L : S.launches)
         if (L.d1 < 0 && (rc = run_launch(ctx, sI, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
+    // the FP16 cross-tile operands of the closure: shifted copies of the finished S2
+    if (ctx->run_half && (rc = timed(ctx, sI, BPMAX_K_FILL, 0, [&] { return launch_s2_shift(sI, dP, npairs, arena); })) !=
+                             BPMAX_OK)
+        return rc;
     HIPCHK(ctx, hipEventRecord(ev_s2, sI));
 
     int o_g = -1, o_src = -2, o_wait = -1; // waits issued on sO for the closure of (group, o_src)
@@ -1579,7 +1583,14 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
             fh += al((tri_ll(d.M) * d.tstride + 1) / 2);
         }
     ctx->run_half = half;
-    ctx->fill_half = fh;
+    ctx->fill_half = fh; // the F mirrors start at -inf; the S2 copies below are written whole
+    if (half)
+        for (auto &d : P) {
+            d.S2r = fl + fh;
+            fh += al((d.tstride + 1) / 2);
+            d.S2c = fl + fh;
+            fh += al((d.tstride + 1) / 2);
+        }
     int rc;
     char *codes_dev = nullptr, *meta_dev = nullptr;
     size_t meta_room = 0;
@@ -1758,8 +1769,16 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
         if ((rc = timed(ctx, st, BPMAX_K_S1, 0, [&] { return launch_s1(st, dP, npairs, arena, dcodes, dw); })) !=
             BPMAX_OK)
             return rc;
-        for (const Launch &L : S.launches)
+        bool shifted = !ctx->run_half;
+        for (const Launch &L : S.launches) {
+            if (!shifted && L.d1 >= 0) { // S2 is done: its shifted FP16 copies for the closure
+                if ((rc = timed(ctx, st, BPMAX_K_FILL, 0, [&] { return launch_s2_shift(st, dP, npairs, arena); })) !=
+                    BPMAX_OK)
+                    return rc;
+                shifted = true;
+            }
             if ((rc = run_launch(ctx, st, L, dP, dI, arena, dcodes, dw, ctx->run_half)) != BPMAX_OK) return rc;
+        }
     }
     // the host vectors (codes, staging) are context members, alive until the next run
     if (!sync) return BPMAX_OK;

@@ -34,6 +34,9 @@ struct PairDesc {
     long long S1;      // float offset of dense S1
     long long Fh = -1; // float offset of the FP16 mirror of F (same tiled layout, one half
                        // per cell; written when the inner closure finalizes a tile), or -1
+    long long S2r = -1; // float offsets of FP16 copies of S2 shifted by one row (S2r[x][y] =
+    long long S2c = -1; // S2[x+1][y]) and by one column (S2c[x][y] = S2[x][y-1]), the cross-tile
+                        // operands of R1 and R2 (same tiled layout; -inf outside S2), or -1
     long long a;       // int offset of seq1 codes
     long long b;       // int offset of reversed, padded seq2 codes
     long long flag;    // int offset of the pair's dependency counters (task scheduler):

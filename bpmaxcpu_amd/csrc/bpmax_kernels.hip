@@ -644,6 +644,68 @@ __device__ void ext_accumulate(const PairDesc &P, float *X, int I, int J, int sb
     __syncthreads();
 }
 
+// FP16 form of ext_accumulate for F (pairs with FP16 mirrors, T = 64): the same steps, every
+// operand a plain logical tile -- F tiles from the mirror, S2 from its row-shifted (R1's B) or
+// column-shifted (R2's A) FP16 copy -- staged straight into LDS three steps ahead (four
+// stages), one barrier per step; wave w takes the 8 k of chunk w of every step on the whole
+// 32 x 32 tile (4 x 4 cells per lane, packed), and the four partial tiles meet in LDS at the
+// end.  `out` as ext_accumulate's; smem (>= 4 XStage) is free again on return.
+static_assert(4 * XStage::BYTES <= 4 * Xk<32>::RED_FLOATS && 4 * XStage::BYTES <= 4 * Intra<32>::FLOATS &&
+                  4 * 512 * 4 <= 4 * XStage::BYTES,
+              "four FP16 cross-tile stages (and the packed partial tiles) fit the closure kernels' LDS");
+__device__ void ext_accumulate_h(const PairDesc &P, int i1, int j1, int I, int J, int sb, int se, float4 &out,
+                                 float *arena, char *smem)
+{
+    constexpr int NS = 4;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
+    const int r0 = 4 * (l >> 3), c0 = 4 * (l & 7), nb = P.nb, D = J - I;
+    const h16bits *Xh = (const h16bits *)(arena + P.Fh) + utri_ll(i1, j1, P.M) * P.tstride;
+    const h16bits *Sr = (const h16bits *)(arena + P.S2r), *Sc = (const h16bits *)(arena + P.S2c);
+    auto lt = [&](const h16bits *tri, int x, int y) {
+        return tri + utri_ll(x >> 1, y >> 1, nb) * 4096 + (x & 1) * 32 * 64 + (y & 1) * 32;
+    };
+    unsigned acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = 0xFC00FC00u;
+    auto issue = [&](int s) {
+        char *st = smem + (s % NS) * XStage::BYTES;
+        if (s < D) { // R1: X(I, I+s) + S2 rows shifted by one (tile (I+s, J) of S2r)
+            x_stage(st, lt(Xh, I, I + s), lt(Sr, I + s, J), w, l);
+        } else { // R2: S2 columns shifted by one (tile (I, K2) of S2c) + X(K2, J)
+            const int K2 = I + 1 + (s - D);
+            x_stage(st, lt(Sc, I, K2), lt(Xh, K2, J), w, l);
+        }
+    };
+    for (int q = 0; q < NS - 1 && sb + q < se; ++q) issue(sb + q);
+    for (int s = sb; s < se; ++s) {
+        // every wave issues one load per step: those of the later steps already in flight may stay
+        const int later = se - 1 - s < NS - 2 ? se - 1 - s : NS - 2;
+        if (later >= 2) vm_wait_le2();
+        else if (later == 1) vm_wait_le1();
+        else vm_wait_all();
+        __syncthreads();
+        if (s + NS - 1 < se) issue(s + NS - 1); // into the stage of step s-1
+        x_prod8(acc, smem + (s % NS) * XStage::BYTES, r0, c0, w);
+    }
+    unsigned *red = (unsigned *)smem; // partial tiles: [wave][row][16 pairs]
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        red[w * 512 + (r0 + i) * 16 + c0 / 2] = acc[i][0];
+        red[w * 512 + (r0 + i) * 16 + c0 / 2 + 1] = acc[i][1];
+    }
+    __syncthreads();
+    const int row = threadIdx.x >> 3, col = 4 * (threadIdx.x & 7);
+    unsigned m0 = red[row * 16 + col / 2], m1 = red[row * 16 + col / 2 + 1];
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+        pk_max3_into(m0, red[q * 512 + row * 16 + col / 2], m0);
+        pk_max3_into(m1, red[q * 512 + row * 16 + col / 2 + 1], m1);
+    }
+    out = make_float4(pk_lo(m0), pk_hi(m0), pk_lo(m1), pk_hi(m1));
+    __syncthreads();
+}
+
 // Always max-combined into the tile (other items of the same tile run concurrently).
 template <int T, bool MODE_S>
 __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restrict__ arena, float *smem)
@@ -651,7 +713,10 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
     constexpr int LT = Inner<T>::LT;
     float *X = MODE_S ? arena + P.S2 : tri_at(arena, P, it.i1, it.j1);
     float4 m;
-    ext_accumulate<T, MODE_S>(P, X, it.I, it.J, it.step_begin, it.step_end, m, arena, smem);
+    if (T == 64 && !MODE_S && P.S2r >= 0)
+        ext_accumulate_h(P, it.i1, it.j1, it.I, it.J, it.step_begin, it.step_end, m, arena, (char *)smem);
+    else
+        ext_accumulate<T, MODE_S>(P, X, it.I, it.J, it.step_begin, it.step_end, m, arena, smem);
     // through LDS, so that each wave instruction max-combines whole row segments (LT
     // consecutive floats) instead of one float of every fourth column
     const int row = threadIdx.x / (LT / 4), col = 4 * (threadIdx.x % (LT / 4));
@@ -722,7 +787,12 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     float4 emax = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
     const bool late_steps = kLate && late && J > I;
     if constexpr (kLate) {
-        if (late_steps) ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, emax, arena, smem);
+        if (late_steps) {
+            if (T == 64 && P.S2r >= 0)
+                ext_accumulate_h(P, i1, j1, I, J, J - I - 1, J - I + 1, emax, arena, (char *)smem);
+            else
+                ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, emax, arena, smem);
+        }
     }
 
     // tiles: every global load issued before any LDS store (float4, unrolled, branch-free;
@@ -1375,6 +1445,32 @@ __global__ void k_fill2(float *__restrict__ p, long long n, float v, long long n
     }
 }
 
+// FP16 copies of S2 shifted by one row and by one column (PairDesc::S2r, S2c), the plain-tile
+// cross-tile operands of the FP16 closure: block (x-chunk, pair), grid-stride over the cells
+// of the pair's upper storage tiles.  S2 values outside the upper triangle read as -inf.
+__global__ void k_s2_shift(const PairDesc *__restrict__ pairs, const float *__restrict__ arena,
+                           float *__restrict__ arena_w)
+{
+    const PairDesc P = pairs[blockIdx.y];
+    if (P.S2r < 0) return;
+    const int T = P.T, nb = P.nb, TT = T * T, n = nb * T;
+    const float *S = arena + P.S2;
+    auto val = [&](int x, int y) -> float {
+        if (y < 0 || x > y || x >= n || y >= n) return NEG_INF;
+        return S[utri_ll(x / T, y / T, nb) * TT + (x % T) * T + y % T];
+    };
+    h16bits *Sr = (h16bits *)(arena_w + P.S2r), *Sc = (h16bits *)(arena_w + P.S2c);
+    const long long cells = tri_ll(nb) * TT;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < cells; e += (long long)gridDim.x * blockDim.x) {
+        const int t = (int)(e / TT), rc = (int)(e % TT), r = rc / T, c = rc % T;
+        int I = 0;
+        while (utri_ll(I + 1, I + 1, nb) <= t) ++I;
+        const int J = I + (t - (int)utri_ll(I, I, nb)), x = I * T + r, y = J * T + c;
+        Sr[e] = __builtin_bit_cast(h16bits, (_Float16)val(x + 1, y));
+        Sc[e] = __builtin_bit_cast(h16bits, (_Float16)val(x, y - 1));
+    }
+}
+
 // F (tiled) -> packed tri-tri rows: block (i2, outer triangle t).
 __global__ void k_to_packed(const float *__restrict__ arena, PairDesc P, float *__restrict__ out)
 {
@@ -1670,6 +1766,12 @@ hipError_t launch_fill(hipStream_t st, float *p, long long n, float v)
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_s2_shift(hipStream_t st, const PairDesc *pairs, int npairs, float *arena)
+{
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_s2_shift, dim3(64, npairs), dim3(256), 0, st, pairs, arena, arena);
     return hipGetLastError();
 }
 hipError_t launch_fill2(hipStream_t st, float *p, long long n, float v, long long n2, unsigned v2)

@@ -25,6 +25,8 @@ hipError_t launch_tasks(hipStream_t st, int T, const PairDesc *pairs, const Work
 hipError_t launch_s1(hipStream_t st, const PairDesc *pairs, int npairs, float *arena, const int *codes,
                      const float *w);
 hipError_t launch_fill(hipStream_t st, float *p, long long n, float v);
+// FP16 row- and column-shifted copies of S2 for the pairs that have them (PairDesc::S2r, S2c)
+hipError_t launch_s2_shift(hipStream_t st, const PairDesc *pairs, int npairs, float *arena);
 // p[0, n) = v, then n2 words of the bit pattern v2
 hipError_t launch_fill2(hipStream_t st, float *p, long long n, float v, long long n2, unsigned v2);
 hipError_t launch_to_packed(hipStream_t st, const float *arena, const PairDesc &P, float *out);

@@ -600,4 +600,52 @@ __device__ __forceinline__ void h_product(unsigned (&acc)[4][2], const char *st,
     for (int k = lo; k < hi; k += 8) h_prod8<RS>(acc, st, rows, c0, k);
 }
 
+// ------------------------------------------ FP16 cross-tile steps (32 x 32 logical tiles)
+// One step of R1/R2 across logical tiles with the S2 operand taken from its shifted FP16 copy
+// (PairDesc::S2r / S2c), so that both operands are plain logical tiles.  Stage of one step
+// (bytes): A two blocks of 16 rows x 4 chunks of 16 B (8 k), block b at b*ABLK (1088: the
+// second block 64 B further into the 256 B bank window), chunk c of block row rr at slot
+// 4*rr + (c ^ ((rr >> 2) & 3)); B 32 rows x 64 B.  Waves 0 and 1 stage A's blocks, waves 2
+// and 3 B's halves, one 16 B load per lane.
+struct XStage {
+    static constexpr int ABLK = 1088, A = 0, B = 2 * ABLK, BYTES = B + 32 * 64;
+};
+
+__device__ __forceinline__ void x_stage(char *st, const h16bits *At, const h16bits *Bt, int w, int l)
+{
+    if (w < 2) {
+        const int rr = l >> 2, c = (l & 3) ^ ((rr >> 2) & 3);
+        g2lds16(At + (16 * w + rr) * 64 + 8 * c, st + XStage::A + w * XStage::ABLK);
+    } else {
+        g2lds16(Bt + (16 * (w - 2) + (l >> 2)) * 64 + 8 * (l & 3), st + XStage::B + (w - 2) * 1024);
+    }
+}
+
+// acc[i][p] (columns c0+2p, c0+2p+1) max= A[r0+i][k] + B[k][c] over the 8 k of chunk kc.
+__device__ __forceinline__ void x_prod8(unsigned (&acc)[4][2], const char *st, int r0, int c0, int kc)
+{
+    uint4 a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + i, rr = r & 15;
+        a[i] = *(const uint4 *)(st + XStage::A + (r >> 4) * XStage::ABLK + 64 * rr + 16 * (kc ^ ((rr >> 2) & 3)));
+    }
+    uint2 b[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = *(const uint2 *)(st + XStage::B + (8 * kc + j) * 64 + 2 * c0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const unsigned av = m == 0 ? a[i].x : m == 1 ? a[i].y : m == 2 ? a[i].z : a[i].w;
+            pk_max3_into(acc[i][0], pk_add_lo(av, b[2 * m].x), pk_add_hi(av, b[2 * m + 1].x));
+            pk_max3_into(acc[i][1], pk_add_lo(av, b[2 * m].y), pk_add_hi(av, b[2 * m + 1].y));
+        }
+    }
+}
+// at most one (two) of the wave's vector-memory operations outstanding: with loads completing
+// in order, every load but the wave's last one (two) has landed
+__device__ __forceinline__ void vm_wait_le1() { asm volatile("s_waitcnt vmcnt(1)" : : : "memory"); }
+__device__ __forceinline__ void vm_wait_le2() { asm volatile("s_waitcnt vmcnt(2)" : : : "memory"); }
+
 } // namespace bpx
