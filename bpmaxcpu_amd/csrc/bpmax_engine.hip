@@ -1435,7 +1435,9 @@ extern "C" int bpmax_schedule_check(int npairs, const int *Ms, const int *Ns, co
                 for (int st = w.step_begin; st < w.step_end; ++st) {
                     const int k1 = outer_k1(w, st, nK), K = Kb + st % nK;
                     if (k1 < w.i1 || k1 >= w.j1 || K > w.J) return fail(std::string("R0 step outside the triangle") + where);
-                    cov[(size_t)(k1 - w.i1) * (w.J - w.I + 1) + (K - w.I)] = 1;
+                    char &c = cov[(size_t)(k1 - w.i1) * (w.J - w.I + 1) + (K - w.I)];
+                    if (c) return fail(std::string("an R0 step is run twice") + where);
+                    c = 1;
                     if (k1s.empty() || k1s.back() != k1) k1s.push_back(k1);
                 }
                 for (int k1 : k1s)

@@ -365,7 +365,7 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
             const int c = c0 >> 3, off = 2 * (c0 & 7);
 #pragma unroll
             for (int i = 0; i < R; ++i) {
-                const uint2 v = *(const uint2 *)(st + rows.base[i] + 16 * (c ^ rows.sw[i]) + off);
+                const uint2 v = *(const uint2 *)(st + rows.base + 128 * i + ((16 * c) ^ rows.sw16) + off);
                 const unsigned t0 = pk_add(v.x, s1b), t1 = pk_add(v.y, s1b);
                 pk_max3_into(acc[i][0], t0, t0);
                 pk_max3_into(acc[i][1], t1, t1);
@@ -764,7 +764,7 @@ __device__ void ext_body(const PairDesc &P, const WorkItem &it, float *__restric
 //
 // BLK (F, LT*LT/4 lanes): the blocked form below instead of the push wavefront.
 template <int T, bool MODE_S, int NT = kThreads, bool BLK = false>
-__device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
+__device__ __forceinline__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, float *__restrict__ arena,
                            const int *__restrict__ codes, const float *__restrict__ w, float *smem,
                            bool late = false)
 {
@@ -781,18 +781,16 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
     float *X = MODE_S ? S : tri_at(arena, P, i1, j1);
     const int tid = threadIdx.x;
 
-    // late cross-tile steps: staged in the LDS that the set-up below overwrites, results
-    // kept in registers until the tile is in LDS
+    // late cross-tile steps: staged in the LDS that the set-up below overwrites, results kept
+    // in registers until the tile is in LDS.  On FP16 operands they run while the tile loads
+    // below are in flight (nothing else writes tile (I,J) during the launch, so the values
+    // loaded stay current); the FP32 form runs first, as its registers and the tile's do not
+    // fit together.
     constexpr bool kLate = !MODE_S && BLK;
     float4 emax = make_float4(NEG_INF, NEG_INF, NEG_INF, NEG_INF);
-    const bool late_steps = kLate && late && J > I;
+    const bool late_steps = kLate && late && J > I, late_h = T == 64 && P.S2r >= 0;
     if constexpr (kLate) {
-        if (late_steps) {
-            if (T == 64 && P.S2r >= 0)
-                ext_accumulate_h(P, i1, j1, I, J, J - I - 1, J - I + 1, emax, arena, (char *)smem);
-            else
-                ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, emax, arena, smem);
-        }
+        if (late_steps && !late_h) ext_accumulate<T, false>(P, X, I, J, J - I - 1, J - I + 1, emax, arena, smem);
     }
 
     // tiles: every global load issued before any LDS store (float4, unrolled, branch-free;
@@ -813,6 +811,9 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         bl = ltile<T, LT>(X, I + 1, J, nb)[tid];
         lf = ltile<T, LT>(X, I, J - 1, nb)[tid * T + LT - 1];
         if (tid == 0 && J - I >= 2) cn = ltile<T, LT>(X, I + 1, J - 1, nb)[LT - 1];
+    }
+    if constexpr (kLate) {
+        if (late_steps && late_h) ext_accumulate_h(P, i1, j1, I, J, J - I - 1, J - I + 1, emax, arena, (char *)smem);
     }
     const int *bc = codes + P.b;
     int vcr = 0, vcc = 0;
@@ -907,22 +908,31 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
         const float *pTL = p2at(a0 + 1, b0 - 1), *pBL = p2at(a0 + 2, b0 - 1), *pBR = p2at(a0 + 2, b0);
         const float sl_in = SL[a0 * L + a0], sr_in = SR[(b0 + 1) * L + b0 + 1]; // partners inside the block
         const int u0 = diag ? 0 : 1 - HB;
+        const int oX = (int)(Xs - smem), oSR = (int)(SR - smem), oSL = (int)(SL - smem);
         for (int u = u0; u < HB; ++u) {
+            // offsets (into smem) of step u's operands, set before its barrier so that the
+            // reads after it issue at once
+            const int k0 = 2 * (A + u - 1);  // row factors: columns k0, k0+1 (>= -LT)
+            const int k1 = 2 * (Bc - u + 1); // column factors: rows k1, k1+1 (> a0+1)
+            int o_xr = oX + a0 * LX + k0;
+            int o_sr = oSR + clampi(k0 + 1, 0, LT - 1) * L + b0; // SR[k0+1][b0..]
+            int o_xc0 = oX + min(k1, LT) * LX + b0, o_xc1 = oX + min(k1 + 1, LT) * LX + b0;
+            int o_sl = oSL + a0 * L + min(k1 - 1, L - 2);
+            asm volatile("" : "+v"(o_xr), "+v"(o_sr), "+v"(o_xc0), "+v"(o_xc1), "+v"(o_sl));
             __syncthreads(); // block diagonal u-1 is final
             if (!active || u > town) continue;
+            // the P2 factors serve the closing step (u == town) only, and are final by then;
+            // read with the step's other operands, so that a step makes one LDS round trip
+            const float p2bl = *pBL, p2tl = *pTL, p2br = *pBR;
             if (u > u0) {
-                const int k0 = 2 * (A + u - 1);  // row factors: columns k0, k0+1 (>= -LT)
-                const int k1 = 2 * (Bc - u + 1); // column factors: rows k1, k1+1 (> a0+1)
-                const float2 xr0 = *(const float2 *)(Xs + a0 * LX + k0);
-                const float2 xr1 = *(const float2 *)(Xs + (a0 + 1) * LX + k0);
-                const int rs = clampi(k0 + 1, 0, LT - 1);
-                const float2 sr0 = *(const float2 *)(SR + rs * L + b0);       // SR[k0+1][b0..]
-                const float2 sr1 = *(const float2 *)(SR + (rs + 1) * L + b0); // SR[k0+2][b0..]
-                const float2 xc0 = *(const float2 *)(Xs + min(k1, LT) * LX + b0);
-                const float2 xc1 = *(const float2 *)(Xs + min(k1 + 1, LT) * LX + b0);
-                const int cs = min(k1 - 1, L - 2);
-                const float sl00 = SL[a0 * L + cs], sl01 = SL[a0 * L + cs + 1];
-                const float sl10 = SL[(a0 + 1) * L + cs], sl11 = SL[(a0 + 1) * L + cs + 1];
+                const float2 xr0 = *(const float2 *)(smem + o_xr);
+                const float2 xr1 = *(const float2 *)(smem + o_xr + LX);
+                const float2 sr0 = *(const float2 *)(smem + o_sr);     // SR[k0+1][b0..]
+                const float2 sr1 = *(const float2 *)(smem + o_sr + L); // SR[k0+2][b0..]
+                const float2 xc0 = *(const float2 *)(smem + o_xc0);
+                const float2 xc1 = *(const float2 *)(smem + o_xc1);
+                const float sl00 = smem[o_sl], sl01 = smem[o_sl + 1];
+                const float sl10 = smem[o_sl + L], sl11 = smem[o_sl + L + 1];
                 max3_into(acc[0][0], xr0.x + sr0.x, xr0.y + sr1.x);
                 max3_into(acc[0][1], xr0.x + sr0.y, xr0.y + sr1.y);
                 max3_into(acc[1][0], xr1.x + sr0.x, xr1.y + sr1.x);
@@ -933,9 +943,9 @@ __device__ void intra_body(const PairDesc &P, int i1, int j1, int I, int J, floa
                 max3_into(acc[1][1], sl10 + xc0.y, sl11 + xc1.y);
             }
             if (u == town) {
-                const float xbl = fmaxf(acc[1][0], *pBL + wv[1][0]);
-                const float xtl = fmaxf(max3(acc[0][0], sl_in + xbl, NEG_INF), *pTL + wv[0][0]);
-                const float xbr = fmaxf(max3(acc[1][1], xbl + sr_in, NEG_INF), *pBR + wv[1][1]);
+                const float xbl = fmaxf(acc[1][0], p2bl + wv[1][0]);
+                const float xtl = fmaxf(max3(acc[0][0], sl_in + xbl, NEG_INF), p2tl + wv[0][0]);
+                const float xbr = fmaxf(max3(acc[1][1], xbl + sr_in, NEG_INF), p2br + wv[1][1]);
                 const float xtr = fmaxf(max3(acc[0][1], xtl + sr_in, sl_in + xbr), xbl + wv[0][1]);
                 *(float2 *)(Xs + a0 * LX + b0) = make_float2(xtl, xtr);
                 *(float2 *)(Xs + (a0 + 1) * LX + b0) = make_float2(xbl, xbr);

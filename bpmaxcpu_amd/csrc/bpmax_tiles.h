@@ -553,31 +553,29 @@ __device__ __forceinline__ void h_stage_B(char *st, const h16bits *Bt, int w, in
         g2lds16(Bt + (16 * w + 8 * q + (l >> 3)) * 64 + 8 * (l & 7), st + HStage::B + (2 * w + q) * 1024);
 }
 
-// Per-lane A addressing of rows r0 .. r0+3: block row base and swizzle.
+// Per-lane A addressing of rows r0 .. r0+3 (r0 a multiple of 4, so the four rows lie in one
+// 16-row block, 128 B apart, and share one swizzle): row r0's base and 16 x its swizzle.
+// Chunk c of row r0+i is at base + 128 i + ((16 c) ^ sw16).
 struct HRows {
-    int base[4], sw[4];
+    int base, sw16;
 };
 __device__ __forceinline__ HRows h_rows(int r0)
 {
-    HRows h;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = r0 + i, rr = r & 15;
-        h.base[i] = HStage::A + (r >> 4) * HStage::ABLK + 128 * rr;
-        h.sw[i] = (rr >> 2) & 3;
-    }
-    return h;
+    const int rr = r0 & 15;
+    return HRows{HStage::A + (r0 >> 4) * HStage::ABLK + 128 * rr, 16 * ((rr >> 2) & 3)};
 }
 
 // acc[i][p] (columns c0+2p, c0+2p+1 packed) max= A[r0+i][k] + B[k+RS][c] over the 8 k from
 // kq (a multiple of 8): four 16 B A reads, eight 8 B B reads, 96 VALU for 128 cell updates.
+// The four sums of a row's two accumulators are formed before either maximum, so that no
+// maximum reads a sum of the instruction just before it (that would cost a wait state).
 template <int RS>
 __device__ __forceinline__ void h_prod8(unsigned (&acc)[4][2], const char *st, const HRows &rows, int c0, int kq)
 {
-    const int c = kq >> 3;
+    const char *ar = st + rows.base + ((2 * kq) ^ rows.sw16); // chunk kq / 8
     uint4 a[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = *(const uint4 *)(st + rows.base[i] + 16 * (c ^ rows.sw[i]));
+    for (int i = 0; i < 4; ++i) a[i] = *(const uint4 *)(ar + 128 * i);
     uint2 b[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) b[j] = *(const uint2 *)(st + HStage::B + (kq + RS + j) * 128 + 2 * c0);
@@ -586,8 +584,10 @@ __device__ __forceinline__ void h_prod8(unsigned (&acc)[4][2], const char *st, c
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const unsigned av = m == 0 ? a[i].x : m == 1 ? a[i].y : m == 2 ? a[i].z : a[i].w;
-            pk_max3_into(acc[i][0], pk_add_lo(av, b[2 * m].x), pk_add_hi(av, b[2 * m + 1].x));
-            pk_max3_into(acc[i][1], pk_add_lo(av, b[2 * m].y), pk_add_hi(av, b[2 * m + 1].y));
+            const unsigned t0 = pk_add_lo(av, b[2 * m].x), t1 = pk_add_hi(av, b[2 * m + 1].x);
+            const unsigned t2 = pk_add_lo(av, b[2 * m].y), t3 = pk_add_hi(av, b[2 * m + 1].y);
+            pk_max3_into(acc[i][0], t0, t1);
+            pk_max3_into(acc[i][1], t2, t3);
         }
     }
 }
@@ -638,8 +638,10 @@ __device__ __forceinline__ void x_prod8(unsigned (&acc)[4][2], const char *st, i
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const unsigned av = m == 0 ? a[i].x : m == 1 ? a[i].y : m == 2 ? a[i].z : a[i].w;
-            pk_max3_into(acc[i][0], pk_add_lo(av, b[2 * m].x), pk_add_hi(av, b[2 * m + 1].x));
-            pk_max3_into(acc[i][1], pk_add_lo(av, b[2 * m].y), pk_add_hi(av, b[2 * m + 1].y));
+            const unsigned t0 = pk_add_lo(av, b[2 * m].x), t1 = pk_add_hi(av, b[2 * m + 1].x);
+            const unsigned t2 = pk_add_lo(av, b[2 * m].y), t3 = pk_add_hi(av, b[2 * m + 1].y);
+            pk_max3_into(acc[i][0], t0, t1); // (sums first: see h_prod8)
+            pk_max3_into(acc[i][1], t2, t3);
         }
     }
 }
