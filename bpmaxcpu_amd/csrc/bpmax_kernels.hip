@@ -190,16 +190,33 @@ __device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__re
         return (I == nb - 1 && I * T + (x >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (x & 1) * QS >= P.N);
     };
     const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
+    // operand tiles of half h: as outer_body_h (one-K items keep their tile offsets and take
+    // k1 without a division; triangle indices in 32 bits; P read once)
+    auto tile_off = [&](int a, int b) { return utri_ll(a, b, nb) * (long long)(T * T); };
+    auto tri32 = [&](int a, int b) { return a * M - (a * (a - 1) >> 1) + (b - a); };
+    const long long tstride = P.tstride;
+    const float *F = arena + P.F;
+    const long long oIK = tile_off(I, Kb), oKJ = tile_off(Kb, J), oK1J = Kb + 1 <= J ? tile_off(Kb + 1, J) : 0;
     auto issue = [&](int h) {
-        const int s = h >> 1, kh = h & 1, k1 = outer_k1(it, s, nK), K = Kb + s % nK;
+        const int s = h >> 1, kh = h & 1;
+        int k1, K;
+        long long aoff = oIK, boff = oKJ, b1off = oK1J;
+        if (nK == 1) {
+            k1 = (it.flags & WI_EDGE) ? (s ? j1 - 1 : i1) : i1 + s;
+            K = Kb;
+        } else {
+            k1 = outer_k1(it, s, nK);
+            K = Kb + s % nK;
+            aoff = tile_off(I, K);
+            boff = tile_off(K, J);
+            b1off = K + 1 <= J ? tile_off(K + 1, J) : 0;
+        }
         float *st = smem + (h & 1) * DlStage::FLOATS;
-        const float *Bf = tri_at(arena, P, k1 + 1, j1);
-        dl_stage_A(st, tri_at(arena, P, i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), kh, wv, l);
-        dl_stage_B32(st, Bf + utri_ll(K, J, nb) * (long long)(T * T), kh, wv, l);
+        const float *Bf = F + (long long)tri32(k1 + 1, j1) * tstride;
+        dl_stage_A(st, F + (long long)tri32(i1, k1) * tstride + aoff, kh, wv, l);
+        dl_stage_B32(st, Bf + boff, kh, wv, l);
         if (wv == 0 && l < 16) { // B row 32 of the half: row 32 of tile K, or row 0 of tile K+1 (-inf past J)
-            const float *src = kh == 0 ? Bf + utri_ll(K, J, nb) * (long long)(T * T) + 32 * T
-                                       : (K + 1 <= J ? Bf + utri_ll(K + 1, J, nb) * (long long)(T * T)
-                                                     : (const float *)kNegInfRow);
+            const float *src = kh == 0 ? Bf + boff + 32 * T : (K + 1 <= J ? Bf + b1off : (const float *)kNegInfRow);
             g2lds16(src + 4 * l, st + DlStage::B + 32 * 64);
         }
         if (kh == 0 && l == 0) {
@@ -211,7 +228,7 @@ __device__ void outer_body_dl(const PairDesc &P, const WorkItem &it, float *__re
     if (h0 < h1) issue(h0);
     float s1a = NEG_INF, s1b = NEG_INF;
     for (int h = h0; h < h1; ++h) {
-        const int s = h >> 1, kh = h & 1, K = Kb + s % nK;
+        const int s = h >> 1, kh = h & 1, K = nK == 1 ? Kb : Kb + s % nK;
         const float *st = smem + (h & 1) * DlStage::FLOATS;
         vm_wait_all();
         __syncthreads();
@@ -312,7 +329,6 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
     const long long toff = utri_ll(I, J, nb) * (long long)(T * T);
     const float *S1 = arena + P.S1;
     const h16bits *Fh = (const h16bits *)(arena + P.Fh);
-    auto tri_h = [&](int a, int b) { return Fh + utri_ll(a, b, M) * P.tstride; };
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64;
     const int kcnt = (it.flags >> WI_KCNT_SHIFT) & 255;
     const int nK = kcnt ? kcnt : J - I + 1, Kb = I + ((it.flags >> WI_KOFF_SHIFT) & 255);
@@ -335,15 +351,31 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
         return (I == nb - 1 && I * T + (x >> 1) * QS >= P.N) || (J == nb - 1 && J * T + (x & 1) * QS >= P.N);
     };
     const bool own_pad = quad_pad(wv), help_pad = g.hw >= 0 && quad_pad(g.hw);
+    // operand tiles of step s.  Items of one K (the usual case) keep their tile offsets and
+    // take k1 without a division; triangle indices in 32 bits (M * M < 2^31)
+    auto tile_off = [&](int a, int b) { return utri_ll(a, b, nb) * (long long)(T * T); };
+    auto tri32 = [&](int a, int b) { return a * M - (a * (a - 1) >> 1) + (b - a); };
+    const long long tstride = P.tstride; // (P lives in memory: read once, not per step)
+    const long long oIK = tile_off(I, Kb), oKJ = tile_off(Kb, J), oK1J = Kb + 1 <= J ? tile_off(Kb + 1, J) : 0;
     auto issue = [&](int s) {
-        const int k1 = outer_k1(it, s, nK), K = Kb + s % nK;
+        int k1, K;
+        long long aoff = oIK, boff = oKJ, b1off = oK1J;
+        if (nK == 1) {
+            k1 = (it.flags & WI_EDGE) ? (s ? j1 - 1 : i1) : i1 + s;
+            K = Kb;
+        } else {
+            k1 = outer_k1(it, s, nK);
+            K = Kb + s % nK;
+            aoff = tile_off(I, K);
+            boff = tile_off(K, J);
+            b1off = K + 1 <= J ? tile_off(K + 1, J) : 0;
+        }
         char *st = smem + (s & 1) * HStage::BYTES;
-        const h16bits *Bf = tri_h(k1 + 1, j1);
-        h_stage_A(st, tri_h(i1, k1) + utri_ll(I, K, nb) * (long long)(T * T), wv, l);
-        h_stage_B(st, Bf + utri_ll(K, J, nb) * (long long)(T * T), wv, l);
+        const h16bits *Bf = Fh + (long long)tri32(k1 + 1, j1) * tstride;
+        h_stage_A(st, Fh + (long long)tri32(i1, k1) * tstride + aoff, wv, l);
+        h_stage_B(st, Bf + boff, wv, l);
         if (wv == 0 && l < 8) { // B row 64: row 0 of tile K+1 (-inf past J)
-            const h16bits *src =
-                K + 1 <= J ? Bf + utri_ll(K + 1, J, nb) * (long long)(T * T) : (const h16bits *)kNegInfRowH;
+            const h16bits *src = K + 1 <= J ? Bf + b1off : (const h16bits *)kNegInfRowH;
             g2lds16(src + 8 * l, st + HStage::B + 64 * 128);
         }
         if (l == 0) {
@@ -353,7 +385,7 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
     };
     if (it.step_begin < it.step_end) issue(it.step_begin);
     for (int s = it.step_begin; s < it.step_end; ++s) {
-        const int K = Kb + s % nK;
+        const int K = nK == 1 ? Kb : Kb + s % nK;
         const char *st = smem + (s & 1) * HStage::BYTES;
         vm_wait_all();
         __syncthreads();
