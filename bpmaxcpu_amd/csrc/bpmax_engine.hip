@@ -1063,7 +1063,16 @@ static int run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const Pai
 int bpx_run_launch(bpmax_ctx *ctx, hipStream_t st, const Launch &L, const PairDesc *dP, const WorkItem *dI,
                    float *arena, const int *dcodes, const float *dw)
 {
-    return run_launch(ctx, st, L, dP, dI, arena, dcodes, dw, false); // server pairs have no FP16 mirror
+    if (L.kind != BPMAX_K_OUTER || L.half_items <= 0) return run_launch(ctx, st, L, dP, dI, arena, dcodes, dw, false);
+    // items of pairs with FP16 tables first, on FP16 operands; the rest on FP32 ones
+    Launch h = L, f = L;
+    h.count = L.half_items;
+    h.updates = L.updates * L.half_items / L.count;
+    f.first = L.first + L.half_items;
+    f.count = L.count - L.half_items;
+    f.updates = L.updates - h.updates;
+    const int rc = run_launch(ctx, st, h, dP, dI, arena, dcodes, dw, true);
+    return rc != BPMAX_OK ? rc : run_launch(ctx, st, f, dP, dI, arena, dcodes, dw, false);
 }
 
 // Dependency events of the pipelined schedule: fill, S2 done, end, then per T group one

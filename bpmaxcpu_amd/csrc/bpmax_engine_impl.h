@@ -29,6 +29,8 @@ struct Launch {
                          // cross-tile steps (k_inner_closure_blk)
     bool bulk = false;   // R0 launch of the k1 terms that read only the triangles two or more
                          // outer diagonals below (runs one outer diagonal ahead)
+    int half_items = 0;  // R0 launch of the serving loop: its first half_items items belong to
+                         // pairs with FP16 tables and run on FP16 operands
 };
 
 // launch-schedule settings of one problem class (bpmax_tuning)

@@ -64,10 +64,11 @@ struct Request {
     bool resident = false;     // the extent is allocated
     PairDesc desc{};           // layout inside the arena (offsets from the arena base)
     int d1 = 0;                // outer diagonal of the pair's next tick
+    bool half = false;         // R0 and the cross-tile steps on FP16 operands (exact for it)
 };
 
 // Operations of one tick, in issue order, on stream 0 (R0 / prologue) or 1 (closure).
-enum OpKind { OP_UPLOAD, OP_CODES, OP_FILL, OP_S1, OP_LAUNCH, OP_RECORD, OP_WAIT, OP_SCORES, OP_DONE };
+enum OpKind { OP_UPLOAD, OP_CODES, OP_FILL, OP_S1, OP_LAUNCH, OP_RECORD, OP_WAIT, OP_SCORES, OP_DONE, OP_S2SHIFT };
 struct Op {
     int kind, stream, a; // a: launch index, joiner index or a count
     long long ev;        // event key (OP_RECORD / OP_WAIT)
@@ -107,16 +108,25 @@ static long long c3ll(long long n) { return n * (n - 1) * (n - 2) / 6; }
 static size_t al64(size_t x) { return (x + 63) / 64 * 64; }
 static size_t round4k(size_t x) { return (x + 4095) / 4096 * 4096; }
 
-// Floats of one pair's extent: F, S2, S1 (as bpx_run_pairs lays them out) and the codes
-// (seq1, then reversed seq2 padded to whole tiles) as ints.
-static size_t extent_floats(int M, int N)
+// FP16 tables of a pair whose R0 runs on FP16 operands, in floats: the mirror of F and the
+// shifted copies of S2 (PairDesc::Fh, S2r, S2c).
+static size_t half_floats(int M, int N)
 {
     const int T = choose_tile(N), nb = (N + T - 1) / T;
-    return pair_device_floats(M, N, T) + al64((size_t)M) + al64((size_t)nb * T);
+    const size_t ts = (size_t)tri_ll(nb) * T * T;
+    return al64((size_t)tri_ll(M) * ts / 2) + 2 * al64(ts / 2);
+}
+
+// Floats of one pair's extent: F, S2, S1 (as bpx_run_pairs lays them out), the FP16 tables
+// when `half`, and the codes (seq1, then reversed seq2 padded to whole tiles) as ints.
+static size_t extent_floats(int M, int N, bool half)
+{
+    const int T = choose_tile(N), nb = (N + T - 1) / T;
+    return pair_device_floats(M, N, T) + (half ? half_floats(M, N) : 0) + al64((size_t)M) + al64((size_t)nb * T);
 }
 
 // Descriptor of a pair whose extent starts at float offset e of the arena.
-static PairDesc layout_extent(int M, int N, long long e)
+static PairDesc layout_extent(int M, int N, long long e, bool half)
 {
     PairDesc d{};
     d.M = M;
@@ -130,6 +140,12 @@ static PairDesc layout_extent(int M, int N, long long e)
     d.S2 = d.F + (long long)al64((size_t)(tri_ll(M) * d.tstride));
     d.S1 = d.S2 + (long long)al64((size_t)d.tstride);
     d.a = d.S1 + (long long)al64((size_t)M * M);
+    if (half) {
+        d.Fh = d.a;
+        d.S2r = d.Fh + (long long)al64((size_t)(tri_ll(M) * d.tstride / 2));
+        d.S2c = d.S2r + (long long)al64((size_t)(d.tstride / 2));
+        d.a = d.S2c + (long long)al64((size_t)(d.tstride / 2));
+    }
     d.b = d.a + (long long)al64((size_t)M);
     return d;
 }
@@ -165,6 +181,7 @@ struct ServeCore {
     long long cap_items = 0;
     size_t cap_codes = 0;        // ints of joiner codes one tick can stage
     bool keep_tables = false;
+    bool r0_half = true;         // FP16 operands for the pairs they are exact for
     Tuning eff{};
     float w[50];
 
@@ -218,7 +235,10 @@ struct ServeCore {
     // Checks that a pair can ever be admitted; fills its footprint.
     int prepare(Request &r, std::string *why) const
     {
-        r.bytes = round4k(extent_floats(r.M, r.N) * sizeof(float));
+        // FP16 tables where exact, unless they would make the pair too big for the budget
+        r.half = r0_half && choose_tile(r.N) == 64 && weights_half_exact(w, w + 25, r.M, r.N) &&
+                 round4k(extent_floats(r.M, r.N, true) * sizeof(float)) <= budget;
+        r.bytes = round4k(extent_floats(r.M, r.N, r.half) * sizeof(float));
         const int T = choose_tile(r.N), nb = (r.N + T - 1) / T;
         r.items = item_bound(r.M, r.N, eff.closure_ahead != 0);
         if (r.bytes > budget) {
@@ -263,7 +283,7 @@ struct ServeCore {
             r.resident = true;
             r.state = RQ_RUNNING;
             r.d1 = 0;
-            r.desc = layout_extent(r.M, r.N, (long long)(off / sizeof(float)));
+            r.desc = layout_extent(r.M, r.N, (long long)(off / sizeof(float)), r.half);
             items_sum += r.items;
             codes_used += ci;
             nbmax = std::max(nbmax, nb);
@@ -344,6 +364,13 @@ struct ServeCore {
             TickGroup &G = tp.groups[g];
             emit_tick(P, tp.d1s, g, G.T, G.nbmax, G.nblmax, (int)tp.tick, tp.S, G.out_end, eff.outer_split,
                       eff.outer_items, eff.outer_kgroup, eff.closure_ahead != 0);
+            // R0 items of pairs with FP16 tables first (in order), so that a launch runs them
+            // on FP16 operands and the rest on FP32 ones
+            for (Launch &L : tp.S.launches) {
+                if (L.group != g || L.d1 < 0 || L.kind != BPMAX_K_OUTER || L.half_items) continue;
+                auto b = tp.S.items.begin() + L.first, e = b + L.count;
+                L.half_items = (int)(std::stable_partition(b, e, [&](const WorkItem &w) { return P[w.pair].Fh >= 0; }) - b);
+            }
             std::set<int> rec;
             for (const Launch &L : tp.S.launches)
                 if (L.group == g && L.d1 >= 0 && L.kind == BPMAX_K_OUTER)
@@ -362,6 +389,10 @@ struct ServeCore {
         const int nl = (int)tp.S.launches.size();
         for (int k = 0; k < nl; ++k)
             if (tp.S.launches[k].d1 < 0) ops.push_back(Op{OP_LAUNCH, 0, k, 0});
+        // the shifted FP16 copies of the joiners' S2 (cross-tile operands of their closure)
+        bool shift = false;
+        for (int j = 0; j < tp.njoin; ++j) shift = shift || P[j].S2r >= 0;
+        if (shift) ops.push_back(Op{OP_S2SHIFT, 0, tp.njoin, 0});
         // R0 ranges, each after the closure of the previous tick reached what it reads
         for (int g = 0; g < (int)tp.groups.size(); ++g) {
             const TickGroup &G = tp.groups[g];
@@ -564,11 +595,13 @@ static int exec_tick(Server &S, const TickPlan &tp, std::string *err)
                                   hipMemcpyHostToDevice, s));
             break;
         }
-        case OP_FILL: {
+        case OP_FILL: { // -inf over F, S2, S1 and (FP16 -inf) over the FP16 tables
             const PairDesc &p = tp.P[o.a];
-            SRVCHK(launch_fill(s, arena + p.F, p.a - p.F, -INFINITY));
+            if (p.Fh >= 0) SRVCHK(launch_fill2(s, arena + p.F, p.Fh - p.F, -INFINITY, p.a - p.Fh, 0xFC00FC00u));
+            else SRVCHK(launch_fill(s, arena + p.F, p.a - p.F, -INFINITY));
             break;
         }
+        case OP_S2SHIFT: SRVCHK(launch_s2_shift(s, dPj, o.a, arena)); break;
         case OP_S1: SRVCHK(launch_s1(s, dPj, o.a, arena, dcodes, dw)); break;
         case OP_LAUNCH: {
             const int rc = bpx_run_launch(c, s, tp.S.launches[o.a], dP, dI, arena, dcodes, dw);
@@ -749,6 +782,7 @@ extern "C" int bpmax_server_start(bpmax_ctx *ctx, const bpmax_server_config *cfg
     const Tuning eff{BPMAX_SCHED_PIPE, ctx->outer_split, ctx->outer_items, 0, ctx->intra_wide, ctx->outer_kgroup,
                      ctx->closure_ahead};
     core_setup(S->core, cfg.memory_budget, cfg.max_batch, cfg.slots, eff, cfg.keep_tables != 0, intra, inter);
+    S->core.r0_half = ctx->r0_half; // (bpmax_ctx_set_r0_half, BPMAX_R0_HALF)
     S->cfg.intra = S->core.w;
     S->cfg.inter = S->core.w + 25;
     int rc = bpmax_ctx_create(&S->eng, ctx->device);
@@ -1052,7 +1086,7 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
     }
     using Key = std::tuple<long long, int, int, int, int>;
     std::map<Key, int> closed, r0_last, ext_at;
-    std::map<long long, int> fill_op, s1_op, s2_last, done_op;
+    std::map<long long, int> fill_op, s1_op, s2_last, done_op, shift_op;
     std::map<long long, std::vector<int>> touch; // ops that run items of a request
     long long nchk = 0;
     for (size_t p = 0; p < plans.size(); ++p) {
@@ -1064,6 +1098,9 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
             if (o.kind == OP_CODES) touch[tp.ids[o.a]].push_back(i);
             if (o.kind == OP_S1)
                 for (int j = 0; j < o.a; ++j) s1_op[tp.ids[j]] = i;
+            if (o.kind == OP_S2SHIFT)
+                for (int j = 0; j < o.a; ++j)
+                    if (tp.P[j].S2r >= 0) shift_op[tp.ids[j]] = i;
             if (o.kind == OP_DONE)
                 for (int f : tp.fin) done_op[tp.ids[f]] = i;
             if (o.kind != OP_LAUNCH) continue;
@@ -1087,7 +1124,7 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
         for (size_t q = 0; q < tp.ops.size(); ++q) {
             const int i = first_op[p] + (int)q;
             const Op &o = tp.ops[q];
-            if (o.kind == OP_S1 || o.kind == OP_FILL || o.kind == OP_CODES) {
+            if (o.kind == OP_S1 || o.kind == OP_FILL || o.kind == OP_CODES || o.kind == OP_S2SHIFT) {
                 if (!before(first_op[p], i) && q > 0) return fail("prologue before the tick's upload");
                 continue;
             }
@@ -1131,6 +1168,11 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
                 }
                 const auto r0 = r0_last.find(Key{id, it.i1, it.j1, it.I / rs, it.J / rs});
                 if (r0 == r0_last.end() || !before(r0->second, i)) return fail(std::string("closure before R0") + where);
+                if (d.S2r >= 0) { // FP16 cross-tile operands: S2's shifted copies
+                    const auto sh = shift_op.find(id);
+                    if (sh == shift_op.end() || !before(s2_last.at(id), sh->second) || !before(sh->second, i))
+                        return fail(std::string("closure before the shifted copies of S2") + where);
+                }
                 const bool cross = L.kind == BPMAX_K_INNER_EXT || wi_type(it.flags) == TASK_EXT;
                 const int D = it.J - it.I;
                 auto tile_closed = [&](int X, int Y) {

@@ -131,7 +131,8 @@ int bpmax_engine_solve_noalloc(bpmax_ctx *ctx, const int *seq1, int M, const int
  * wavefront for all resident pairs at once, each on its own diagonal -- and enqueues the
  * ticks on two streams of its own, `slots` ticks ahead of the device.  Submitted pairs queue
  * FIFO and join at the next tick that has room: room in the arena for the pair's extent (its
- * tiled F layout, S1, S2 and codes, ~M(M+1)N(N+1)/4 x 4 B, bpmax_engine_pair_device_bytes),
+ * tiled F layout, S1, S2 and codes, ~M(M+1)N(N+1)/4 x 4 B, bpmax_engine_pair_device_bytes;
+ * half the bytes of F more for a pair with FP16 R0 operands, see bpmax_ctx_set_r0_half),
  * fewer than `max_batch` resident pairs, and room in the tick's work-item staging.  Requests
  * that arrive while others run therefore start while those run, and pairs leave after their
  * last diagonal.  Every device buffer is allocated by bpmax_server_start; the serving loop
@@ -317,9 +318,10 @@ int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);
  * writes each finished F tile as FP16 into a mirror (half the bytes of F, extra device
  * memory) and the R0 launches multiply those on packed halves (v_pk_add_f16 +
  * v_pk_maximum3_f16: 4 cell updates per 3 VALU instructions).  Every value is then exact,
- * so F and the score are bit-identical either way; other solves, the serving loop and the
- * no-allocation solve use FP32 operands.  BPMAX_R0_HALF=0 in the environment sets the
- * default.  No counterpart in the reference (its R0 is FP32 AVX2). */
+ * so F and the score are bit-identical either way; other solves and the no-allocation
+ * solve use FP32 operands.  The serving loop chooses per pair with the same bound (the
+ * setting of the context at bpmax_server_start).  BPMAX_R0_HALF=0 in the environment sets
+ * the default.  No counterpart in the reference (its R0 is FP32 AVX2). */
 int bpmax_ctx_set_r0_half(bpmax_ctx *ctx, int enable);
 /* 1 if the last solve of the context ran its R0 launches on FP16 operands, else 0. */
 int bpmax_ctx_r0_half_used(const bpmax_ctx *ctx);
