@@ -791,12 +791,13 @@ static void emit_closure(const std::vector<PairDesc> &P, const std::vector<int> 
         // closure ahead: steps [0, D) and [D+2, 2D+2) of the tiles on D+1, in items longer
         // than the cross-tile launches' (these share launches with the in-tile work; 64x512,
         // 24x512 and 48x384 ran faster with 8..32 steps than with 4 or with the cross-tile
-        // launches' chunks)
+        // launches' chunks, and with FP16 cross-tile steps 64x512 ran 1 % faster again with
+        // at least 16: 39.3 ms against 39.7 with 8; 24x512, 48x384 and 16x256 within 0.5 %)
         const int D1 = D + 1;
         long long tot = 0;
         for (size_t p = 0; p < P.size() && closure_ahead; ++p)
             if (active(p) && P[p].nblr > D1) tot += (long long)(P[p].M - d1s[p]) * (P[p].nblr - D1) * 2 * D;
-        const int ch = (int)std::min<long long>(32, std::max<long long>(8, tot / 1024));
+        const int ch = (int)std::min<long long>(32, std::max<long long>(16, tot / 1024));
         const int tex = (TASK_EXT << WI_TYPE_SHIFT) | WI_ATOMIC;
         // by triangle, each triangle's in-tile items then its cross-tile items, so that every
         // XCD's contiguous share of the launch mixes the two kinds
