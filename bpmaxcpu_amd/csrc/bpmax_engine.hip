@@ -92,7 +92,13 @@ bool weights_exact(const float *intra, const float *inter, int M, int N, std::st
 // integer multiple of 2^-p no larger than wmax * ((M+N)/2 + 1), so with that bound times 2^p
 // at most 2^11 every score, and every R0 sum (never above the final value of its cell), is an
 // exact FP16 number.  Weights that pass weights_exact only.
-bool weights_half_exact(const float *intra, const float *inter, int M, int N)
+//
+// With the sequences (s1, s2 non-null) and no negative weight, a tighter bound: each base
+// pairs at most once and every term of the recurrence adds one pair's weight or sums scores
+// over disjoint ranges, so every score is at most the sum over the bases of half the best
+// weight each base can pair with (w(x,y) <= (best(x) + best(y)) / 2).  For uniform ACGU and
+// the default table that is 1.06 per base instead of 1.5: M + N up to ~960 instead of 680.
+bool weights_half_exact(const float *intra, const float *inter, int M, int N, const int *s1, const int *s2)
 {
     int p = -1;
     for (int cand = 0; cand <= 12 && p < 0; ++cand) {
@@ -103,9 +109,26 @@ bool weights_half_exact(const float *intra, const float *inter, int M, int N)
         }
         if (ok) p = cand;
     }
-    float wmax = 0.f;
-    for (int t = 0; t < 25; ++t) wmax = std::max(wmax, std::max(intra[t], inter[t]));
-    return p >= 0 && (double)wmax * (((double)M + N) / 2.0 + 1.0) * std::ldexp(1.0, p) <= 2048.0;
+    if (p < 0) return false;
+    float wmax = 0.f, wmin = 0.f;
+    for (int t = 0; t < 25; ++t) {
+        wmax = std::max(wmax, std::max(intra[t], inter[t]));
+        wmin = std::min(wmin, std::min(intra[t], inter[t]));
+    }
+    const double scale = std::ldexp(1.0, p);
+    if ((double)wmax * (((double)M + N) / 2.0 + 1.0) * scale <= 2048.0) return true;
+    if (!s1 || !s2 || wmin < 0.f) return false;
+    double best[5]; // best weight of a pair with base code x, either table, either order
+    for (int x = 0; x < 5; ++x) {
+        best[x] = 0.0;
+        for (int y = 0; y < 5; ++y)
+            best[x] = std::max({best[x], (double)intra[5 * x + y], (double)intra[5 * y + x], (double)inter[5 * x + y],
+                                (double)inter[5 * y + x]});
+    }
+    double bound = 0.0;
+    for (int i = 0; i < M; ++i) bound += best[s1[i]] / 2.0;
+    for (int i = 0; i < N; ++i) bound += best[s2[i]] / 2.0;
+    return bound * scale <= 2048.0;
 }
 
 } // namespace bpx
@@ -1587,7 +1610,7 @@ static int run_pairs(bpmax_ctx *ctx, int npairs, const int *const *seq1s, const 
     // FP16 (the R0 launches then read them; the no-allocation mode keeps its scratch for FP32)
     bool half = ctx->r0_half && !ctx->carve && eff.schedule != BPMAX_SCHED_TASKS && ctx->schedule != BPMAX_SCHED_TASKS;
     for (int p = 0; p < npairs && half; ++p)
-        half = P[p].T == 64 && weights_half_exact(wt, wt + 25, P[p].M, P[p].N);
+        half = P[p].T == 64 && weights_half_exact(wt, wt + 25, P[p].M, P[p].N, seq1s[p], seq2s[p]);
     long long fh = 0;
     if (half)
         for (auto &d : P) {

@@ -69,7 +69,8 @@ int choose_tile(int N);
 int pick_tile(int N, int tile);
 size_t pair_device_floats(int M, int N, int T);
 bool weights_exact(const float *intra, const float *inter, int M, int N, std::string *why);
-bool weights_half_exact(const float *intra, const float *inter, int M, int N);
+bool weights_half_exact(const float *intra, const float *inter, int M, int N, const int *s1 = nullptr,
+                        const int *s2 = nullptr);
 // r0_ahead: the bulk of each outer diagonal's R0 (k1 terms reading only triangles two or
 // more diagonals below) is emitted one diagonal early, for the R0 stream, and the two edge
 // k1 terms with the closure (see enqueue_pipe); otherwise R0 of d1 is emitted whole.

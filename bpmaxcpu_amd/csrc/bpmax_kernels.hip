@@ -336,7 +336,7 @@ __device__ void outer_body_h(const PairDesc &P, const WorkItem &it, float *__res
 #pragma unroll
     for (int i = 0; i < R; ++i) acc[i][0] = acc[i][1] = acc2[i][0] = acc2[i][1] = 0xFC00FC00u;
     const bool one_k = BAL && nK == 1 && it.step_begin < it.step_end;
-    const int map = !one_k || Kb == I && Kb == J ? HMAP_QUAD : (Kb == I ? HMAP_ROWS : (Kb == J ? HMAP_COLS : HMAP_QUAD));
+    const int map = !one_k || (Kb == I && Kb == J) ? HMAP_QUAD : (Kb == I ? HMAP_ROWS : (Kb == J ? HMAP_COLS : HMAP_QUAD));
     const OuterRanges g = map == HMAP_QUAD ? outer_ranges<T, BAL>(it, nK, Kb, wv) : h_band_ranges(map, wv);
     int r0, c0, hr0 = 0, hc0 = 0;
     h_cells(map, wv, l, r0, c0);

@@ -236,7 +236,7 @@ struct ServeCore {
     int prepare(Request &r, std::string *why) const
     {
         // FP16 tables where exact, unless they would make the pair too big for the budget
-        r.half = r0_half && choose_tile(r.N) == 64 && weights_half_exact(w, w + 25, r.M, r.N) &&
+        r.half = r0_half && choose_tile(r.N) == 64 && weights_half_exact(w, w + 25, r.M, r.N, r.s1.data(), r.s2.data()) &&
                  round4k(extent_floats(r.M, r.N, true) * sizeof(float)) <= budget;
         r.bytes = round4k(extent_floats(r.M, r.N, r.half) * sizeof(float));
         const int T = choose_tile(r.N), nb = (r.N + T - 1) / T;
@@ -987,8 +987,10 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
             r.M = Ms[k];
             r.N = Ns[k];
             if (r.M < 1 || r.N < 1) return fail("bad shape");
-            r.s1.assign(r.M, 0);
-            r.s2.assign(r.N, 0);
+            // all G: the largest best pair weight per base under the default table, so the
+            // FP16 choice follows the length-only bound
+            r.s1.assign(r.M, 2);
+            r.s2.assign(r.N, 2);
             std::string why;
             if (C.prepare(r, &why) != BPMAX_OK) return fail(why);
             rid[k++] = C.submit(std::move(r));
@@ -1015,8 +1017,8 @@ extern "C" int bpmax_server_check(int nreq, const int *Ms, const int *Ns, const 
                 Request r;
                 r.M = Ms[q];
                 r.N = Ns[q];
-                r.s1.assign(r.M, 0);
-                r.s2.assign(r.N, 0);
+                r.s1.assign(r.M, 2);
+                r.s2.assign(r.N, 2);
                 std::string why;
                 if (C.prepare(r, &why) != BPMAX_OK) return fail(why);
                 rid[q] = C.submit(std::move(r));
