@@ -86,6 +86,7 @@ def load_library(path: str = LIB_PATH):
         "bpmax_ctx_stream": (c_void_p, [c_void_p]),
         "bpmax_ctx_synchronize": (c_int, [c_void_p]),
         "bpmax_cell_updates": (ctypes.c_double, [c_int, c_int]),
+        "bpmax_score_bound": (ctypes.c_double, [c_void_p, c_void_p, ip, c_int, ip, c_int]),
         "bpmax_base_code": (ctypes.c_ubyte, [ctypes.c_char]),
         "bpmax_read_sequence_file": (c_int, [ctypes.c_char_p, ip, c_int]),
         "bpmax_write_checking_dump": (c_int, [ctypes.c_char_p, fp, c_int, c_int, c_int]),
@@ -210,6 +211,22 @@ def encode(seq: str) -> np.ndarray:
 def cell_updates(M: int, N: int) -> float:
     """Cell updates of the full recurrence: C(M+1,3)C(N+1,3)+2C(M+1,2)C(N+1,3)+2C(M+1,3)C(N+1,2)."""
     return float(load_library().bpmax_cell_updates(M, N))
+
+
+def score_bound(seq1, seq2, intra=None, inter=None) -> float:
+    """Upper bound on every score of the pair (bpmax_score_bound; host only).  R0 runs on FP16
+    operands when it times 2^p is at most 2^11 (weights multiples of 2^-p)."""
+    s1 = np.ascontiguousarray(seq1, np.int32)
+    s2 = np.ascontiguousarray(seq2, np.int32)
+    ti = None if intra is None else np.ascontiguousarray(intra, np.float32).reshape(25)
+    te = None if inter is None else np.ascontiguousarray(inter, np.float32).reshape(25)
+    ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    ip = ctypes.POINTER(ctypes.c_int)
+    b = load_library().bpmax_score_bound(ptr(ti), ptr(te), s1.ctypes.data_as(ip), len(s1), s2.ctypes.data_as(ip),
+                                         len(s2))
+    if b < 0:
+        raise BPMaxError(EINVAL, "score_bound: empty or out-of-range sequence, or a negative or non-finite weight")
+    return float(b)
 
 
 def _table(t, default):

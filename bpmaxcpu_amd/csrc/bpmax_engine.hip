@@ -95,43 +95,162 @@ bool weights_exact(const float *intra, const float *inter, int M, int N, std::st
 //
 // With the sequences (s1, s2 non-null) and no negative weight, a tighter bound: each base
 // pairs at most once and every term of the recurrence adds one pair's weight or sums scores
-// over disjoint ranges, so every score is at most the sum over the bases of half the best
-// weight each base can pair with (w(x,y) <= (best(x) + best(y)) / 2).  For uniform ACGU and
-// the default table that is 1.06 per base instead of 1.5: M + N up to ~960 instead of 680.
-bool weights_half_exact(const float *intra, const float *inter, int M, int N, const int *s1, const int *s2)
+// over disjoint ranges, so a score is the weight of a matching of the pair's bases, with
+// W(x,y) = the larger weight of codes x and y in either table and order.  Any y >= 0 with
+// y[x] + y[z] >= W(x,z) for all codes x, z (x = z included) bounds every matching by
+// sum over bases of y[code]; y = best/2 (best[x] = max_z W(x,z)) is one such, and lowering
+// one y[x] at a time to the least value the others allow keeps the property.  Tried in turn,
+// each only when the one before fails: best/2; the least sum after lowering in every order
+// of the codes; the least sum over every vertex of that polyhedron, which is the minimum
+// (equal to the largest fractional matching; for the default table 1.0 per base for
+// uniform ACGU, M + N up to 1024, against 1.5 per base by length).  A vertex solves 5 of the
+// 20 constraints as equalities; its basis is a forest of one-cycle components (odd cycles,
+// loops, or a tight y >= 0), so it is a multiple of half the weights' grid, 2^-(p+1): each
+// solution is rounded to that grid and kept only if it satisfies every constraint exactly.
+static double vertex_bound(const double (&W)[5][5], const double (&cnt)[5], double grid, double upper)
 {
-    int p = -1;
-    for (int cand = 0; cand <= 12 && p < 0; ++cand) {
-        bool ok = true;
-        for (int t = 0; t < 50 && ok; ++t) {
-            const double s = std::ldexp((double)(t < 25 ? intra[t] : inter[t - 25]), cand);
-            ok = (s == std::floor(s));
+    struct Row { int x, z; double rhs; }; // y[x] + y[z] >= rhs (z = x: 2 y[x]; z < 0: y[x] >= 0)
+    Row rows[20];
+    int nr = 0;
+    for (int x = 0; x < 5; ++x)
+        for (int z = x; z < 5; ++z) rows[nr++] = Row{x, z, W[x][z]};
+    for (int x = 0; x < 5; ++x) rows[nr++] = Row{x, -1, 0.0};
+    double best = upper;
+    int pick[5] = {0, 1, 2, 3, 4};
+    for (;;) {
+        double A[5][6] = {};
+        for (int r = 0; r < 5; ++r) {
+            const Row &q = rows[pick[r]];
+            A[r][q.x] += 1.0;
+            if (q.z >= 0) A[r][q.z] += 1.0;
+            A[r][5] = q.rhs;
         }
-        if (ok) p = cand;
+        bool ok = true;
+        for (int c = 0; c < 5 && ok; ++c) { // Gaussian elimination, partial pivoting
+            int pv = c;
+            for (int r = c + 1; r < 5; ++r)
+                if (std::fabs(A[r][c]) > std::fabs(A[pv][c])) pv = r;
+            if (std::fabs(A[pv][c]) < 1e-9) {
+                ok = false;
+                break;
+            }
+            for (int k = 0; k < 6; ++k) std::swap(A[c][k], A[pv][k]);
+            for (int r = 0; r < 5; ++r) {
+                if (r == c) continue;
+                const double f = A[r][c] / A[c][c];
+                for (int k = c; k < 6; ++k) A[r][k] -= f * A[c][k];
+            }
+        }
+        if (ok) {
+            double y[5];
+            for (int x = 0; x < 5; ++x) y[x] = std::round(A[x][5] / A[x][x] / grid) * grid;
+            for (int r = 0; r < 20 && ok; ++r) {
+                const Row &q = rows[r];
+                ok = (q.z < 0 ? y[q.x] : (q.z == q.x ? 2.0 * y[q.x] : y[q.x] + y[q.z])) >= q.rhs;
+            }
+            if (ok) {
+                double t = 0.0;
+                for (int x = 0; x < 5; ++x) t += cnt[x] * y[x];
+                best = std::min(best, t);
+            }
+        }
+        int i = 4; // next 5-subset of the 20 rows, in lexicographic order
+        while (i >= 0 && pick[i] == 15 + i) --i;
+        if (i < 0) break;
+        ++pick[i];
+        for (int k = i + 1; k < 5; ++k) pick[k] = pick[k - 1] + 1;
     }
-    if (p < 0) return false;
+    return best;
+}
+
+// The smallest of the bounds above on every score of an M x N pair (in score units), trying
+// the next one only while the bound is above `stop`.  p: the weights' grid exponent.
+static double score_bound(const float *intra, const float *inter, int M, int N, const int *s1, const int *s2, int p,
+                          double stop)
+{
     float wmax = 0.f, wmin = 0.f;
     for (int t = 0; t < 25; ++t) {
         wmax = std::max(wmax, std::max(intra[t], inter[t]));
         wmin = std::min(wmin, std::min(intra[t], inter[t]));
     }
-    const double scale = std::ldexp(1.0, p);
-    if ((double)wmax * (((double)M + N) / 2.0 + 1.0) * scale <= 2048.0) return true;
-    if (!s1 || !s2 || wmin < 0.f) return false;
-    double best[5]; // best weight of a pair with base code x, either table, either order
+    double bound = (double)wmax * (((double)M + N) / 2.0 + 1.0);
+    if (bound <= stop || !s1 || !s2 || wmin < 0.f) return bound;
+    double W[5][5], y0[5], cnt[5] = {0, 0, 0, 0, 0};
     for (int x = 0; x < 5; ++x) {
-        best[x] = 0.0;
-        for (int y = 0; y < 5; ++y)
-            best[x] = std::max({best[x], (double)intra[5 * x + y], (double)intra[5 * y + x], (double)inter[5 * x + y],
-                                (double)inter[5 * y + x]});
+        y0[x] = 0.0;
+        for (int z = 0; z < 5; ++z) {
+            W[x][z] = std::max({(double)intra[5 * x + z], (double)intra[5 * z + x], (double)inter[5 * x + z],
+                                (double)inter[5 * z + x]});
+            y0[x] = std::max(y0[x], W[x][z] / 2.0);
+        }
     }
-    double bound = 0.0;
-    for (int i = 0; i < M; ++i) bound += best[s1[i]] / 2.0;
-    for (int i = 0; i < N; ++i) bound += best[s2[i]] / 2.0;
-    return bound * scale <= 2048.0;
+    for (int i = 0; i < M; ++i) cnt[s1[i]] += 1.0;
+    for (int i = 0; i < N; ++i) cnt[s2[i]] += 1.0;
+    auto total = [&](const double *y) {
+        double t = 0.0;
+        for (int x = 0; x < 5; ++x) t += cnt[x] * y[x];
+        return t;
+    };
+    bound = std::min(bound, total(y0));
+    if (bound <= stop) return bound;
+    int order[5] = {0, 1, 2, 3, 4};
+    do {
+        double y[5];
+        std::copy(y0, y0 + 5, y);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int t = 0; t < 5; ++t) {
+                const int x = order[t];
+                double v = W[x][x] / 2.0;
+                for (int z = 0; z < 5; ++z)
+                    if (z != x) v = std::max(v, W[x][z] - y[z]);
+                y[x] = std::max(0.0, v);
+            }
+        bound = std::min(bound, total(y));
+    } while (std::next_permutation(order, order + 5));
+    if (bound <= stop) return bound;
+    return vertex_bound(W, cnt, std::ldexp(0.5, -p), bound);
+}
+
+static int grid_exponent(const float *intra, const float *inter)
+{
+    for (int cand = 0; cand <= 12; ++cand) {
+        bool ok = true;
+        for (int t = 0; t < 50 && ok; ++t) {
+            const double s = std::ldexp((double)(t < 25 ? intra[t] : inter[t - 25]), cand);
+            ok = (s == std::floor(s));
+        }
+        if (ok) return cand;
+    }
+    return -1;
+}
+
+bool weights_half_exact(const float *intra, const float *inter, int M, int N, const int *s1, const int *s2)
+{
+    const int p = grid_exponent(intra, inter);
+    if (p < 0) return false;
+    const double stop = std::ldexp(2048.0, -p);
+    return score_bound(intra, inter, M, N, s1, s2, p, stop) <= stop;
 }
 
 } // namespace bpx
+
+extern "C" double bpmax_score_bound(const float *intra, const float *inter, const int *seq1, int M, const int *seq2,
+                                   int N)
+{
+    if (!seq1 || !seq2 || M < 1 || N < 1) return -1.0;
+    for (int i = 0; i < M; ++i)
+        if (seq1[i] < 0 || seq1[i] > 4) return -1.0;
+    for (int i = 0; i < N; ++i)
+        if (seq2[i] < 0 || seq2[i] > 4) return -1.0;
+    float dflt_intra[25], dflt_inter[25], wt[50];
+    bpmax_engine_score_table(dflt_intra, dflt_inter);
+    memcpy(wt, intra ? intra : dflt_intra, 25 * sizeof(float));
+    memcpy(wt + 25, inter ? inter : (intra ? intra : dflt_inter), 25 * sizeof(float));
+    for (int t = 0; t < 50; ++t)
+        if (!std::isfinite(wt[t]) || wt[t] < 0.f) return -1.0;
+    const int p = grid_exponent(wt, wt + 25);
+    return bpx::score_bound(wt, wt + 25, M, N, seq1, seq2, p < 0 ? 0 : p, -1.0);
+}
 
 // ----------------------------------------------------------------------------- ctx
 

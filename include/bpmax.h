@@ -324,6 +324,16 @@ int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);
  * setting of the context at bpmax_server_start).  BPMAX_R0_HALF=0 in the environment sets
  * the default.  No counterpart in the reference (its R0 is FP32 AVX2). */
 int bpmax_ctx_set_r0_half(bpmax_ctx *ctx, int enable);
+/* An upper bound on every score (every F cell and every partial sum) of the pair seq1 x seq2
+ * (codes 0..4, seq2 in its natural order) under the given tables (NULL: as
+ * bpmax_engine_solve), the least of: wmax * ((M+N)/2 + 1); and, since a score is the weight
+ * of a matching of the pair's bases with W(x,z) the larger weight of codes x, z in either
+ * table and order, the least sum over bases of y[code] over y >= 0 with y[x] + y[z] >= W(x,z)
+ * (the fractional matching optimum, by the vertices of that polyhedron).  R0 runs on FP16
+ * operands for a pair when this bound times 2^p is at most 2^11 (weights multiples of
+ * 2^-p).  Host only, no device call.  -1 for null or out-of-range input or a negative or
+ * non-finite weight.  No counterpart in the reference. */
+double bpmax_score_bound(const float *intra, const float *inter, const int *seq1, int M, const int *seq2, int N);
 /* 1 if the last solve of the context ran its R0 launches on FP16 operands, else 0. */
 int bpmax_ctx_r0_half_used(const bpmax_ctx *ctx);
 /* The hipStream_t the context launches on (as void*). */

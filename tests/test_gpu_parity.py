@@ -198,17 +198,17 @@ def test_invalid_inputs_are_rejected(eng):
 
 def test_r0_operand_precision_follows_the_score_bound(eng):
     """R0 runs on FP16 operands exactly when every score is an FP16 integer multiple of 2^-p
-    and the pair has 64 x 64 tiles: by length, wmax * ((M+N)/2 + 1) * 2^p <= 2048, or, for
-    non-negative weights, by sequence, 2^p times the sum over both strands' bases of half the
-    best weight each base can pair with <= 2048.  F and the score are bit-identical to the
-    oracle either way."""
+    and the pair has 64 x 64 tiles: by length, wmax * ((M+N)/2 + 1) * 2^p <= 2048, or by
+    sequence, 2^p times the pair's matching bound (B.score_bound) <= 2048.  F and the score
+    are bit-identical to the oracle either way."""
     rng = np.random.default_rng(5)
     a, b = rng.integers(0, 4, 16).astype(np.int32), rng.integers(0, 4, 100).astype(np.int32)
     intra = B.reference_table()  # GC 3, AU 1, GU 1.5: p = 1
     inter = intra
     # bounds at 16 x 100 ((16+100)/2 + 1 = 59 pairs): 3 * 59 * 2^1 = 354.  x16 (GU 24, p = 0):
-    # by length 48 * 59 = 2832, by sequence 1888 (this draw: best A 1, C 3, G 3, U 1.5, halved
-    # and summed 118, x16); x32: 3776 by sequence; x0.25: 0.75 * 59 * 2^3 = 354 (p = 3)
+    # by length 48 * 59 = 2832, by sequence 16 * 101 = 1616 (this draw's matching bound is
+    # 101: 23 G, 29 C, 32 A, 32 U); x32: 3232 by sequence; x0.25: 0.75 * 59 * 2^3 = 354 (p = 3)
+    assert B.score_bound(a, b, intra, inter) == 101.0
     cases = [(intra, inter, True), (intra * 16, inter * 16, True), (intra * 32, inter * 32, False),
              (intra * 0.25, inter * 0.25, True)]
     for wi, we, expect in cases:
@@ -233,9 +233,9 @@ def test_r0_operand_precision_follows_the_score_bound(eng):
 def test_fp16_at_the_edge_of_its_exact_range(eng):
     """8 x 674 with scores up to 1018.5 (2037 halves, FP16 holds integers to 2048): GGGGCCCC
     against G^336 C^335 U^3.  The length bound (3 * 342 * 2 = 2052) rules FP16 out; the
-    sequence bound (1.5 per G or C, 0.75 per U: 1020.75, 2041.5 halves) lets it run, and F
+    matching bound (339 G-C + 1 G-U = 1018.5, the score itself: 2037 halves) lets it run, and F
     must equal the oracle's bit for bit, 127 of its cells above 900 being half-integers.  Two
-    more C (8 x 676, all G or C: 2052 halves) is past both bounds: FP32 operands."""
+    more bases (8 x 676, all G or C: 2052 halves) is past both bounds: FP32 operands."""
     a = np.array([2, 2, 2, 2, 1, 1, 1, 1], np.int32)
     b = np.array([2] * 336 + [1] * 335 + [3] * 3, np.int32)
     t = B.reference_table()
