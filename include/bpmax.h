@@ -314,8 +314,8 @@ int bpmax_ctx_set_default_tuning(bpmax_ctx *ctx, const bpmax_tuning *t);
 int bpmax_ctx_set_graphs(bpmax_ctx *ctx, int enable);
 /* Enable (default) or disable FP16 operands for R0.  When every pair of a solve (or batch)
  * has 64 x 64 storage tiles and a score bound that FP16 represents exactly -- the weights
- * are multiples of 2^-p and wmax * ((M+N)/2 + 1) * 2^p <= 2^11, or 2^p times the sum over
- * both strands' bases of half the largest weight each base can pair with is -- the inner
+ * are multiples of 2^-p and bpmax_score_bound (which includes wmax * ((M+N)/2 + 1)) times
+ * 2^p is at most 2^11 -- the inner
  * closure also writes each finished F tile as FP16 into a mirror (half the bytes of F,
  * extra device memory) and the R0 launches multiply those on packed halves (v_pk_add_f16 +
  * v_pk_maximum3_f16: 4 cell updates per 3 VALU instructions).  Every value is then exact,
