@@ -68,6 +68,8 @@ def load_library(path: str = LIB_PATH):
         "bpmax_engine_pair_device_bytes": (c_size_t, [c_int, c_int]),
         "bpmax_tri_tiled_floats": (c_size_t, [c_int]),
         "bpmax_r0_tri_batch_device": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+        "bpmax_r0_tri_batch_device_half": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+        "bpmax_tri_to_half": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, ctypes.POINTER(c_int)]),
         "bpmax_tri_upload": (c_int, [c_void_p, c_int, c_int, fp, c_void_p]),
         "bpmax_tri_download": (c_int, [c_void_p, c_int, c_int, c_void_p, fp]),
         "bpmax_tri_fill_random": (c_int, [c_void_p, c_int, c_int, ctypes.c_ulonglong, c_int, c_int, c_void_p]),
@@ -363,6 +365,18 @@ class Engine:
 
     def r0_tri_batch(self, n: int, batch: int, A: int, B: int, C: int):
         self._check(self.L.bpmax_r0_tri_batch_device(self.h, n, batch, A, B, C))
+
+    def tri_to_half(self, src: int, dst: int, n: int, batch: int) -> bool:
+        """FP16 copies of tiled triangles (dst from tri_alloc holds twice what they need);
+        True when every value is -inf or an integer of magnitude <= 1024, so that the FP16
+        R0 gives exact results (bpmax_tri_to_half)."""
+        exact = ctypes.c_int()
+        self._check(self.L.bpmax_tri_to_half(self.h, n, batch, src, dst, ctypes.byref(exact)))
+        return bool(exact.value)
+
+    def r0_tri_batch_half(self, n: int, batch: int, Ah: int, Bh: int, C: int):
+        """C max= A (max,+) B on FP16 operand copies, C in FP32 (bpmax_r0_tri_batch_device_half)."""
+        self._check(self.L.bpmax_r0_tri_batch_device_half(self.h, n, batch, Ah, Bh, C))
 
     # -------------------------------------------------------------- schedule / timing
     def set_schedule(self, schedule, chunk: int = 0):

@@ -335,6 +335,7 @@ extern "C" int bpmax_ctx_destroy(bpmax_ctx *ctx)
     free_buf(ctx->out);
     free_buf(ctx->scores);
     free_buf(ctx->r0meta);
+    free_buf(ctx->r0half);
     free_buf(ctx->flags);
     (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -2124,17 +2125,17 @@ extern "C" int bpmax_device_free(bpmax_ctx *ctx, void *p)
     return BPMAX_OK;
 }
 
-extern "C" int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const float *A_dev, const float *B_dev,
-                                         float *C_dev)
+// Tile list of the isolated R0 (heavy tiles, long K ranges, first so the tail of the grid is
+// short) into ctx->r0meta, then one timed launch.
+static int r0_tri_run(bpmax_ctx *ctx, int n, int batch,
+                      const std::function<hipError_t(int nb, const int2 *tiles, int ntiles)> &launch)
 {
-    if (!ctx || n < 1 || batch < 1 || !A_dev || !B_dev || !C_dev) return BPMAX_EINVAL;
     if (batch > 65535) {
         ctx->err = "batch must be <= 65535";
         return BPMAX_EINVAL;
     }
     const int T = 64, nb = (n + T - 1) / T;
     std::vector<int2> tiles;
-    // heavy tiles (long K ranges) first so the tail of the grid is short
     for (int D = nb - 1; D >= 0; --D)
         for (int I = 0; I + D < nb; ++I) tiles.push_back(make_int2(I, I + D));
     int rc;
@@ -2143,12 +2144,46 @@ extern "C" int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const
                                ctx->stream));
     const double nn = n;
     const double upd = (double)batch * (nn + 2) * (nn + 1) * nn / 6.0;
-    rc = timed(ctx, ctx->stream, BPMAX_K_R0TRI, upd, [&] {
-        return launch_r0tri(ctx->stream, T, nb, (const int2 *)ctx->r0meta.p, (int)tiles.size(), batch, A_dev, B_dev,
-                            C_dev);
-    });
+    rc = timed(ctx, ctx->stream, BPMAX_K_R0TRI, upd,
+               [&] { return launch(nb, (const int2 *)ctx->r0meta.p, (int)tiles.size()); });
     if (rc != BPMAX_OK) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return bpx_collect_timing(ctx);
+}
+
+extern "C" int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const float *A_dev, const float *B_dev,
+                                         float *C_dev)
+{
+    if (!ctx || n < 1 || batch < 1 || !A_dev || !B_dev || !C_dev) return BPMAX_EINVAL;
+    return r0_tri_run(ctx, n, batch, [&](int nb, const int2 *tiles, int ntiles) {
+        return launch_r0tri(ctx->stream, 64, nb, tiles, ntiles, batch, A_dev, B_dev, C_dev);
+    });
+}
+
+extern "C" int bpmax_r0_tri_batch_device_half(bpmax_ctx *ctx, int n, int batch, const unsigned short *A_dev,
+                                              const unsigned short *B_dev, float *C_dev)
+{
+    if (!ctx || n < 1 || batch < 1 || !A_dev || !B_dev || !C_dev) return BPMAX_EINVAL;
+    return r0_tri_run(ctx, n, batch, [&](int nb, const int2 *tiles, int ntiles) {
+        return launch_r0tri_h(ctx->stream, nb, tiles, ntiles, batch, A_dev, B_dev, C_dev);
+    });
+}
+
+extern "C" int bpmax_tri_to_half(bpmax_ctx *ctx, int n, int batch, const float *src_dev, unsigned short *dst_dev,
+                                 int *exact)
+{
+    if (!ctx || n < 1 || batch < 1 || !src_dev || !dst_dev || !exact) return BPMAX_EINVAL;
+    const long long nb = (n + 63) / 64, elems = (long long)batch * tri_ll((int)nb) * 64 * 64;
+    int rc;
+    if ((rc = bpx_grow(ctx, ctx->r0half, 256)) != BPMAX_OK) return rc;
+    int *flag = (int *)ctx->r0half.p, inexact = 1;
+    HIPCHK(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+    rc = timed(ctx, ctx->stream, BPMAX_K_CONVERT, 0,
+               [&] { return launch_tri_to_half(ctx->stream, elems, src_dev, dst_dev, flag); });
+    if (rc != BPMAX_OK) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(&inexact, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *exact = inexact == 0;
     return bpx_collect_timing(ctx);
 }
 

@@ -95,6 +95,7 @@ struct bpmax_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bpx::DevBuf arena, codes, meta, out, scores, r0meta, flags;
+    bpx::DevBuf r0half;                  // bpmax_tri_to_half's inexact flag
     std::vector<int> flags_host, flags_staging;
     int schedule = BPMAX_SCHED_PIPE;
     int task_chunk = 16;

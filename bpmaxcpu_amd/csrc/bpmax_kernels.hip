@@ -1670,6 +1670,98 @@ __global__ __launch_bounds__(kThreads) void k_r0tri(int nb, const int2 *__restri
     store_acc<T, T>(tile_at(Cm, IJ.x, IJ.y, nb, T), acc, r0, c0, false);
 }
 
+// FP16 copy of isolated R0 operands, four values per lane and step.  A value is exact for the
+// FP16 product when it is -inf or an integer of magnitude <= 1024: every sum of two such values
+// is then -inf or an integer of magnitude <= 2048, which FP16 holds exactly, and so is every
+// maximum of them.  Any other value (a fraction, a larger magnitude, +inf, NaN) sets *inexact.
+__global__ void k_tri_to_half(const float4 *__restrict__ src, long long n4, uint2 *__restrict__ dst,
+                              int *__restrict__ inexact)
+{
+    bool bad = false;
+    auto exact = [](float v) { return v == NEG_INF || (fabsf(v) <= 1024.0f && v == rintf(v)); };
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n4; q += (long long)gridDim.x * blockDim.x) {
+        const float4 v = src[q];
+        bad |= !(exact(v.x) && exact(v.y) && exact(v.z) && exact(v.w));
+        dst[q] = make_uint2(pk_pack(v.x, v.y), pk_pack(v.z, v.w));
+    }
+    if (bad) atomicOr(inexact, 1);
+}
+
+// Isolated R0 on FP16 operands (copies made by k_tri_to_half, or any whose sums FP16 holds): r0tri_dl's work and tile order
+// with a whole 64-deep K step per LDS stage (HStage, one barrier and one load wait per step)
+// and products on packed halves (h_product, 4 cell updates per 3 VALU instructions).  The
+// waves' k ranges on the diagonal steps are balanced_k_range's, the helpers' slices merged on
+// packed halves at the end; C is max-combined in FP32 (its values need not be FP16-exact).
+__global__ __launch_bounds__(kThreads) void k_r0tri_h(int nb, const int2 *__restrict__ tiles,
+                                                      const h16bits *__restrict__ A, const h16bits *__restrict__ B,
+                                                      float *__restrict__ C)
+{
+    constexpr int T = 64, R = 4;
+    __shared__ __attribute__((aligned(16))) char smem[2 * HStage::BYTES];
+    const int wi = xcd_work_index(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int2 IJ = tiles[wi % gridDim.x];
+    const int I = IJ.x, J = IJ.y;
+    const long long tstride = tri_ll(nb) * T * T;
+    const long long m = wi / gridDim.x;
+    const h16bits *Am = A + m * tstride, *Bm = B + m * tstride;
+    float *Ct = C + m * tstride + utri_ll(I, J, nb) * (long long)(T * T);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), l = threadIdx.x % 64, hw = balance_target(wv);
+    int r0, c0, hr0 = 0, hc0 = 0;
+    lane_block<T>(r0, c0);
+    if (hw >= 0) lane_block<T>(hr0, hc0, hw * 64 + l);
+    const HRows rows = h_rows(r0), hrows = h_rows(hr0);
+    unsigned acc[R][2], acc2[R][2];
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc[i][0] = acc[i][1] = acc2[i][0] = acc2[i][1] = 0xFC00FC00u;
+    float4 cv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) cv[i] = *(const float4 *)(Ct + (r0 + i) * T + c0);
+    auto tile = [&](const h16bits *X, int a, int b) { return X + utri_ll(a, b, nb) * (long long)(T * T); };
+    auto issue = [&](int K) {
+        char *st = smem + ((K - I) & 1) * HStage::BYTES;
+        h_stage_A(st, tile(Am, I, K), wv, l);
+        h_stage_B(st, tile(Bm, K, J), wv, l);
+    };
+    issue(I);
+    for (int K = I; K <= J; ++K) {
+        const char *st = smem + ((K - I) & 1) * HStage::BYTES;
+        vm_wait_all();
+        __syncthreads();
+        if (K + 1 <= J) issue(K + 1); // into the stage every wave finished reading before this barrier
+        int kb, ke, hb, he;
+        balanced_k_range<T>(K == I, K == J, wv, kb, ke, hb, he);
+        h_product<0>(acc, st, rows, c0, kb, ke);
+        if (hw >= 0) h_product<0>(acc2, st, hrows, hc0, hb, he);
+    }
+    // helpers hand their slices to their target waves (waves 0, 1 and 3) through LDS
+    unsigned *x = (unsigned *)smem;
+    __syncthreads();
+    if (hw >= 0) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            x[hw * 512 + i * 128 + 2 * l] = acc2[i][0];
+            x[hw * 512 + i * 128 + 2 * l + 1] = acc2[i][1];
+        }
+    }
+    __syncthreads();
+    if (wv != 2) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            pk_max3_into(acc[i][0], x[wv * 512 + i * 128 + 2 * l], acc[i][0]);
+            pk_max3_into(acc[i][1], x[wv * 512 + i * 128 + 2 * l + 1], acc[i][1]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        float4 c = cv[i];
+        c.x = fmaxf(c.x, pk_lo(acc[i][0]));
+        c.y = fmaxf(c.y, pk_hi(acc[i][0]));
+        c.z = fmaxf(c.z, pk_lo(acc[i][1]));
+        c.w = fmaxf(c.w, pk_hi(acc[i][1]));
+        *(float4 *)(Ct + (r0 + i) * T + c0) = c;
+    }
+}
+
 // Dense row-major n x n batch <-> tiled triangles (lower cells -> -inf on both sides).
 __global__ void k_dense_to_tiled(int n, int nb, int T, const float *__restrict__ src, float *__restrict__ dst)
 {
@@ -1847,6 +1939,23 @@ hipError_t launch_r0tri(hipStream_t st, int T, int nb, const int2 *tiles, int nt
 {
     if (ntiles <= 0 || batch <= 0) return hipSuccess;
     BPX_T_DISPATCH(T, hipLaunchKernelGGL(k_r0tri<TT>, dim3(ntiles, batch), dim3(kThreads), 0, st, nb, tiles, A, B, C));
+    return hipGetLastError();
+}
+hipError_t launch_tri_to_half(hipStream_t st, long long n, const float *src, unsigned short *dst, int *inexact)
+{
+    if (n <= 0 || n % 4) return hipErrorInvalidValue;
+    const long long n4 = n / 4;
+    long long blocks = (n4 + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_tri_to_half, dim3((unsigned)blocks), dim3(256), 0, st, (const float4 *)src, n4, (uint2 *)dst,
+                       inexact);
+    return hipGetLastError();
+}
+hipError_t launch_r0tri_h(hipStream_t st, int nb, const int2 *tiles, int ntiles, int batch, const unsigned short *Ah,
+                          const unsigned short *Bh, float *C)
+{
+    if (ntiles <= 0 || batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_r0tri_h, dim3(ntiles, batch), dim3(kThreads), 0, st, nb, tiles, Ah, Bh, C);
     return hipGetLastError();
 }
 hipError_t launch_dense_to_tiled(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst)

@@ -34,6 +34,12 @@ hipError_t launch_to_dense(hipStream_t st, const float *arena, const PairDesc &P
 hipError_t launch_scores(hipStream_t st, const PairDesc *pairs, int npairs, const float *arena, float *scores);
 hipError_t launch_r0tri(hipStream_t st, int T, int nb, const int2 *tiles, int ntiles, int batch, const float *A,
                         const float *B, float *C);
+// FP16 copy of n floats (n a multiple of 4); *inexact is set (never cleared) when a value is
+// not -inf or an integer of magnitude <= 1024
+hipError_t launch_tri_to_half(hipStream_t st, long long n, const float *src, unsigned short *dst, int *inexact);
+// isolated R0 on 64 x 64 tiles from those FP16 copies
+hipError_t launch_r0tri_h(hipStream_t st, int nb, const int2 *tiles, int ntiles, int batch, const unsigned short *Ah,
+                          const unsigned short *Bh, float *C);
 hipError_t launch_dense_to_tiled(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst);
 hipError_t launch_tiled_to_dense(hipStream_t st, int n, int nb, int T, int batch, const float *src, float *dst);
 hipError_t launch_tri_random(hipStream_t st, int n, int nb, int T, int batch, unsigned long long seed, int lo, int hi,

@@ -208,6 +208,21 @@ int bpmax_write_checking_dump(const char *path, const float *F_packed, int M, in
 size_t bpmax_tri_tiled_floats(int n);
 int bpmax_r0_tri_batch_device(bpmax_ctx *ctx, int n, int batch, const float *A_dev,
                               const float *B_dev, float *C_dev);
+/* FP16 copies of `batch` tiled triangles: dst_dev holds batch*bpmax_tri_tiled_floats(n)
+ * 16-bit values.  *exact = 1 when every value is -inf or an integer of magnitude <= 1024;
+ * every sum of two such values is then exact in FP16, and
+ * bpmax_r0_tri_batch_device_half on the copies gives C bit for bit as
+ * bpmax_r0_tri_batch_device on the originals.  *exact = 0 otherwise (the copies are then
+ * rounded toward zero and must not be used for exact results). */
+int bpmax_tri_to_half(bpmax_ctx *ctx, int n, int batch, const float *src_dev,
+                      unsigned short *dst_dev, int *exact);
+/* bpmax_r0_tri_batch_device on FP16 operands (IEEE half bit patterns in the same tiled
+ * layout; C stays FP32 and is combined in FP32), the reference's R0 max-plus micro-kernel
+ * operation (micro_kernels/maxplus/max_plus.cpp) on packed halves.  Exact when every sum of
+ * an A and a B value is an FP16 number, as bpmax_tri_to_half's *exact = 1 guarantees. */
+int bpmax_r0_tri_batch_device_half(bpmax_ctx *ctx, int n, int batch,
+                                   const unsigned short *A_dev, const unsigned short *B_dev,
+                                   float *C_dev);
 /* Host <-> tiled-device conversion helpers for those triples (dense row-major n x n on
  * the host; entries below the diagonal are ignored on upload, -inf on download). */
 int bpmax_tri_upload(bpmax_ctx *ctx, int n, int batch, const float *host_dense, float *dev_tiled);

@@ -253,28 +253,69 @@ def test_fp16_at_the_edge_of_its_exact_range(eng):
     assert sc2 == so2
 
 
+@pytest.mark.parametrize("half", [False, True], ids=["fp32", "fp16"])
 @pytest.mark.parametrize("n,batch", [(1, 1), (5, 3), (64, 2), (65, 2), (130, 3), (512, 2)])
-def test_r0_isolated_vs_oracle(eng, n, batch):
-    """C max= A (max,+) B on tiled triangles against the oracle."""
+def test_r0_isolated_vs_oracle(eng, n, batch, half):
+    """C max= A (max,+) B on tiled triangles against the oracle, on FP32 operands and on their
+    FP16 copies (integers of magnitude <= 1024: every sum below 2^11 is exact in FP16)."""
     rng = np.random.default_rng(n + batch)
     up = np.triu(np.ones((n, n), bool))
     mk = lambda: np.where(up, rng.integers(0, 201, (batch, n, n)), -np.inf).astype(np.float32)  # noqa: E731
     A, Bm, C = mk(), mk(), mk()
-    _say(f"r0 isolated n={n} batch={batch}")
-    dA, dB, dC = eng.tri_alloc(n, batch), eng.tri_alloc(n, batch), eng.tri_alloc(n, batch)
+    if half:  # the whole exact range, negative values included
+        A = np.where(up, rng.integers(-1024, 1025, (batch, n, n)), -np.inf).astype(np.float32)
+    _say(f"r0 isolated n={n} batch={batch} half={half}")
+    bufs = [eng.tri_alloc(n, batch) for _ in range(5 if half else 3)]
+    dA, dB, dC = bufs[:3]
     try:
         eng.tri_upload(A, dA)
         eng.tri_upload(Bm, dB)
         eng.tri_upload(C, dC)
-        eng.r0_tri_batch(n, batch, dA, dB, dC)
+        if half:
+            assert eng.tri_to_half(dA, bufs[3], n, batch) and eng.tri_to_half(dB, bufs[4], n, batch)
+            eng.r0_tri_batch_half(n, batch, bufs[3], bufs[4], dC)
+        else:
+            eng.r0_tri_batch(n, batch, dA, dB, dC)
         got = eng.tri_download(dC, n, batch)
     finally:
-        for p in (dA, dB, dC):
+        for p in bufs:
             eng.tri_free(p)
     for m in range(batch):
         want = O.maxplus_tri(A[m], Bm[m], C[m])
         assert _bits_equal(np.where(up, got[m], 0), np.where(up, want, 0))
         assert np.all(got[m][~up] == -np.inf)
+
+
+@pytest.mark.parametrize("value,exact", [(0.5, False), (1025.0, False), (-1025.0, False), (np.inf, False),
+                                         (1024.0, True), (-1024.0, True), (-np.inf, True)])
+def test_tri_to_half_exactness(eng, value, exact):
+    """bpmax_tri_to_half reports exact copies only when every value is -inf or an integer of
+    magnitude <= 1024: one planted value decides, wherever it sits in the batch; the copy
+    itself holds the values (checked through the FP16 R0 against the oracle when exact)."""
+    n, batch = 130, 2
+    rng = np.random.default_rng(23)
+    up = np.triu(np.ones((n, n), bool))
+    A = np.where(up, rng.integers(-1000, 1001, (batch, n, n)), -np.inf).astype(np.float32)
+    Bm = np.where(up, rng.integers(-1000, 1001, (batch, n, n)), -np.inf).astype(np.float32)
+    C = np.full((batch, n, n), -np.inf, np.float32)
+    A[1, 70, 129] = value
+    _say(f"tri_to_half planted {value}")
+    bufs = [eng.tri_alloc(n, batch) for _ in range(5)]
+    try:
+        eng.tri_upload(A, bufs[0])
+        eng.tri_upload(Bm, bufs[1])
+        eng.tri_upload(C, bufs[2])
+        assert eng.tri_to_half(bufs[0], bufs[3], n, batch) == exact
+        assert eng.tri_to_half(bufs[1], bufs[4], n, batch)
+        if exact:
+            eng.r0_tri_batch_half(n, batch, bufs[3], bufs[4], bufs[2])
+            got = eng.tri_download(bufs[2], n, batch)
+            for m in range(batch):
+                want = O.maxplus_tri(A[m], Bm[m], C[m])
+                assert _bits_equal(np.where(up, got[m], 0), np.where(up, want, 0))
+    finally:
+        for p in bufs:
+            eng.tri_free(p)
 
 
 def _maxplus_tri_np(A, Bm, C):
@@ -309,6 +350,38 @@ def test_r0_isolated_at_full_batch_size(eng):
             assert _bits_equal(got, want), m
     finally:
         for p in (dA, dB, dC):
+            eng.tri_free(p)
+
+
+def test_r0_isolated_fp16_at_full_batch_size(eng):
+    """The FP16-operand R0 at the benchmark size: 4096 tiled 512x512 triangles of synthetic
+    integers in [0, 200], copied to FP16 (exact), the product of every matrix equal bit for
+    bit to the FP32 kernel's on the same operands, four of them also to a numpy max-plus."""
+    n, batch = 512, 4096
+    _say(f"r0 isolated fp16 n={n} batch={batch}")
+    tri = eng.tri_tiled_floats(n)
+    bufs = [eng.tri_alloc(n, batch) for _ in range(4)]
+    dA, dB, dC, dC2 = bufs
+    half = [eng.tri_alloc(n, (batch + 1) // 2) for _ in range(2)]
+    try:
+        for d, seed in ((dA, 1), (dB, 2), (dC, 3), (dC2, 3)):
+            eng.tri_fill_random(d, n, batch, seed, 0, 200)
+        picks = [0, 4095]
+        before = {m: [eng.tri_download(d + 4 * m * tri, n, 1)[0] for d in (dA, dB, dC)] for m in picks}
+        assert eng.tri_to_half(dA, half[0], n, batch) and eng.tri_to_half(dB, half[1], n, batch)
+        eng.r0_tri_batch_half(n, batch, half[0], half[1], dC)
+        eng.r0_tri_batch(n, batch, dA, dB, dC2)
+        for m0 in range(0, batch, 512):  # every matrix, 512 at a time
+            a = eng.tri_download(dC + 4 * m0 * tri, n, 512)
+            b = eng.tri_download(dC2 + 4 * m0 * tri, n, 512)
+            assert _bits_equal(a, b), m0
+            del a, b
+        for m in picks:
+            A, Bm, C = before[m]
+            got = eng.tri_download(dC + 4 * m * tri, n, 1)[0]
+            assert _bits_equal(got, _maxplus_tri_np(A, Bm, C)), m
+    finally:
+        for p in bufs + half:
             eng.tri_free(p)
 
 
