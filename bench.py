@@ -536,6 +536,9 @@ def main():
                          "FP16, two launches each of 2048 workgroups x 4 waves with 32 x 1000 and 32 x 4000 VALU "
                          "instructions per wave, the reference points that scale rocprofv3 SQ_INSTS_VALU to wave "
                          "instructions and measure each kind's issue rate")
+    ap.add_argument("--r0-only", action="store_true",
+                    help="only run the isolated R0 leg (4096 tiled 512x512 triangles, FP32 then FP16 "
+                         "operands) and print its JSON")
     args = ap.parse_args()
     if args.calibrate_valu:
         import ctypes
@@ -573,6 +576,10 @@ def main():
     import bpmaxcpu_amd as B
     M, N = args.M, args.N
     eng = B.Engine(0)
+    if args.r0_only:
+        print(json.dumps({"r0_isolated": r0_leg(B, eng)}), flush=True)
+        eng.close()
+        return
     if args.write_tuning:
         s1, s2 = mixed_pairs()
         table = [{"M": M, "N": N, "pairs": 1, **eng.tune(M, N, 1, reps=3)},
